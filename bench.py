@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""PBF particle-steps/s on one MI355X: 1,048,576-particle dam break, 4 solver
+iterations, XSPH 0.01, marching cubes on the 256^3 density field every step.
+
+One "step" = predict + Morton key + radix sort + reorder + cell table, 4 x
+(lambda, delta), finalise, XSPH, then density-field sampling, case
+classification + count, device scan and triangle emission. Inputs are
+resident in HBM before the timed region; nothing crosses PCIe inside it.
+
+Prints one JSON line. Per-kernel device times come from HIP events recorded
+on the library's own stream around every launch of the timed steps; the
+roofline entry is the kernel with the largest share of device time, priced
+with its algorithmic bytes (the byte model below). The CPU baseline is the
+CPU restatement (oracle/, FP32, OpenMP on the host cores) on the first steps
+of the same scene.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(name, n, G, S, T):
+    """Compulsory HBM bytes of one launch: each input read once, each output
+    written once; vec3 = 12 B, scalars and keys 4 B, type 1 B.
+    n particles, G cell-table entries, S lattice samples, T triangles."""
+    cubes = S  # (S-1)^3 ~ S^3 at 256^3
+    table = {
+        "predict_key": 29 * n + 8 * n,          # x, v, m, type -> key, index
+        "radix_hist": 4 * n,                    # keys
+        "radix_scatter": 8 * n + 8 * n,         # (key, index) in and out
+        "reorder": 4 * n + 33 * n + 4 * n + 33 * n + 12 * n,  # perm, state in/out, p*
+        "cell_table": 4 * n + 4 * G,
+        "pbf_lambda": 21 * n + 4 * n + 4 * G + 4 * n,  # p*, m, type, key, table -> lambda, rho
+        "pbf_lambda_kahan": 21 * n + 4 * n + 4 * G + 4 * n,
+        "pbf_delta": 29 * n + 4 * n + 4 * G,   # p*, lambda, type, key, mass, table -> p*
+        "pbf_finalise": 1 * n + 12 * n + 24 * n + 24 * n,  # type, p*, x, v -> x, v
+        "pbf_xsph": 12 * n + 1 * n + 12 * n + 4 * n + 4 * G + 12 * n,
+        "pbf_diffuse": 1 * n + 4 * n + 4 * n + 4 * G + 4 * n,
+        "mc_density_field": 4 * S + 16 * n + 4 * G,  # field out; particles + table in
+        "mc_classify_count": 4 * S,
+        "mc_emit": 4 * S + 36 * T,
+    }
+    return table.get(name)
+
+
+def cpu_baseline(scene, mc, budget_s=20.0):
+    """The CPU restatement (FP32, OpenMP) on the first steps of the scene,
+    marching cubes included; returns (particle-steps/s, steps, threads)."""
+    import oracle
+    p = scene.params()
+    o = oracle.Pbf(p)
+    o.load(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
+    steps = 0
+    t = 0.0
+    while steps == 0 or (t < budget_s and steps < scene.steps):
+        t0 = time.perf_counter()
+        o.step()
+        field = o.mc_density(mc.samples)
+        origin, step = o.density_lattice(mc.samples)
+        oracle.mc_emit(field, mc.isolevel, origin, step, 1.0)
+        t += time.perf_counter() - t0
+        steps += 1
+        if t > budget_s / 2 and steps >= 1:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return scene.n * steps / t, steps, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--scene", default="dam_break_1m")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import libfluid_amd as lf
+    from libfluid_amd import scenes
+
+    scene = getattr(scenes, args.scene)()
+    p = scene.params()
+    mc = scene.mc()
+    n = scene.n
+    print("bench: %s n=%d steps=%d warmup=%d" % (scene.name, n, args.steps, args.warmup),
+          file=sys.stderr, flush=True)
+
+    ctx = lf.Context(0)
+    ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
+
+    def one_step():
+        ctx.step(p, 1)
+        ctx.mc_sample(p, mc)
+        ctx.mc_classify_count(mc)
+        ctx.mc_emit(p, mc)
+
+    for _ in range(args.warmup):
+        one_step()
+    ctx.sync()
+    ntri_warm = ctx.mc_count()
+
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_step()
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    stats = ctx.kernel_stats()
+    ntri = ctx.mc_count()
+    G = ctx.grid_info()[0]
+    S = int(np.prod(mc.samples))
+
+    value = n * args.steps / elapsed
+    ms_per_step = 1000.0 * elapsed / args.steps
+    device_ms = sum(v[1] for v in stats.values())
+    dom_name, (dom_count, dom_ms) = max(stats.items(), key=lambda kv: kv[1][1])
+    dom_avg_s = dom_ms / dom_count / 1000.0
+    dom_bytes = algorithmic_bytes(dom_name, n, G, S, ntri)
+    achieved = dom_bytes / dom_avg_s / 1e9 if dom_bytes else None
+    kernels = {}
+    for name, (cnt, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        b = algorithmic_bytes(name, n, G, S, ntri)
+        avg = ms / cnt / 1000.0
+        kernels[name] = {"launches": cnt, "avg_us": round(avg * 1e6, 2),
+                         "share": round(ms / device_ms, 4),
+                         "alg_GBs": round(b / avg / 1e9, 1) if b else None}
+
+    out = {
+        "metric": "PBF particle-steps/s @1M, 4 iters (% HBM peak); batched-mixed vs plain FP32",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded jittered lattice dam break)",
+        "config": {"workload": "%s: plain FP32 PBF, %d particles, h=%g, %d iterations, XSPH %g, "
+                               "marching cubes %dx%dx%d density field every step"
+                               % (scene.name, n, scene.h, scene.iterations, scene.xsph_c,
+                                  *mc.samples),
+                   "particles": n, "iterations": scene.iterations,
+                   "mc_samples": list(mc.samples), "triangles_per_step": ntri},
+        "roofline": {"bound": "hbm", "kernel": dom_name,
+                     "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "alg_bytes_per_launch": dom_bytes,
+                     "avg_launch_us": round(dom_avg_s * 1e6, 2)},
+        "kernels": kernels,
+        "device_ms_per_step": device_ms / args.steps,
+    }
+    ctx.close()
+    if not args.no_cpu:
+        cv, csteps, threads = cpu_baseline(scene, mc)
+        out["cpu_baseline"] = {
+            "value": cv, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "first %d step(s) of the same %d-particle scene incl. 256^3 marching "
+                      "cubes, FP32 CPU restatement, OpenMP" % (csteps, n)}
+        out["gpu_over_cpu"] = value / cv
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

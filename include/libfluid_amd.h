@@ -1,0 +1,234 @@
+/*
+ * libfluid_amd — C ABI of the MI355X (gfx950) Position Based Fluids step and
+ * its marching-cubes surface extraction.
+ *
+ * Plain C: opaque context, POD parameter blocks, plain pointers and sizes, an
+ * int status on every call (0 = ok, < 0 = error; lf_last_error() has the
+ * text). No exceptions cross this boundary.
+ *
+ * Each entry point names the reference interface it replaces
+ * (tom91136/libfluid, paths relative to the reference root).
+ *
+ * Host pointers are host memory. Everything the step works on lives in device
+ * memory owned by the context; it persists across calls and grows on demand.
+ * Calls on one context are ordered on the context's own HIP stream; calls that
+ * hand data back to the host synchronise that stream first.
+ */
+#ifndef LIBFLUID_AMD_H
+#define LIBFLUID_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define LF_OK 0
+#define LF_ERR_ARG (-1)        /* bad argument (null pointer, size, ...) */
+#define LF_ERR_HIP (-2)        /* HIP runtime error (text in lf_last_error) */
+#define LF_ERR_MORTON (-3)     /* grid extent >= 1024 cells on an axis: the
+                                  10-bit Morton axis of include/fluid/curves.h:56-72
+                                  would wrap */
+#define LF_ERR_CAPACITY (-4)   /* marching-cubes output exceeded its capacity */
+#define LF_ERR_STATE (-5)      /* call out of order (e.g. lambda before grid) */
+#define LF_ERR_NODEVICE (-6)   /* no HIP device */
+
+/* ---- particle types (include/fluid/fluid.hpp:18-20) --------------------- */
+#define LF_FLUID 0
+#define LF_OBSTACLE 1
+
+/* ---- flags of lf_pbf_params.flags ---------------------------------------- */
+#define LF_FLAG_DIFFUSE 0x1u   /* run the colour diffusion pass
+                                  (sph_diffuse, oclsph_kernel.h:281-303) */
+#define LF_FLAG_KAHAN 0x2u     /* Kahan-compensated density and lambda-denominator
+                                  sums in the lambda pass (mixed precision) */
+
+/*
+ * Parameters of one PBF step. Replaces ClSphConfig (oclsph_type.h:6-13), the
+ * compile-time -DSPH_H (oclsph.hpp:170-178) and the constants hard-coded in
+ * oclsph_kernel.h:20-31; h is a runtime value here.
+ */
+typedef struct lf_pbf_params {
+  float h;              /* kernel radius = grid cell size (SPH_H) */
+  float dt;             /* Config::dt (fluid.hpp:175) */
+  float scale;          /* Config::scale: world = solver units * scale */
+  uint32_t iterations;  /* Config::iteration: lambda/delta passes per step */
+  float gravity[3];     /* Config::constantForce (an acceleration: F = m*g) */
+  float min_bound[3];   /* Config::minBound, world units */
+  float max_bound[3];   /* Config::maxBound, world units */
+  float rest_density;   /* RHO = 6378 (oclsph_kernel.h:21) */
+  float cfm_epsilon;    /* CFM_EPSILON = 600 (oclsph_kernel.h:25) */
+  float corr_k;         /* CorrK = 1e-4 (oclsph_kernel.h:30) */
+  float corr_dq;        /* CorrDeltaQ / h = 0.3 (oclsph_kernel.h:26) */
+  float vd;             /* VD = 0.49, velocity damping (oclsph_kernel.h:20) */
+  float xsph_c;         /* XSPH viscosity c; 0 = off (the reference has none) */
+  uint32_t flags;       /* LF_FLAG_* */
+} lf_pbf_params;
+
+/* Fills p with the reference's constants for kernel radius h (dt 1/120,
+ * scale 1, 4 iterations, no gravity, unit box, XSPH off). */
+void lf_pbf_params_default(lf_pbf_params* p, float h);
+
+/* A gravity well (fluid.hpp:137-144); applied in the predict pass exactly as
+ * oclsph.hpp:306-317 (radius 75 world units, force clamped to +-10). */
+typedef struct lf_well {
+  float centre[3];
+  float force;
+} lf_well;
+
+/* Marching-cubes parameters. Replaces ClMcConfig (oclsph_type.h:39-44). */
+#define LF_MC_FIELD_REFERENCE 0 /* mc_lattice metaball field, lattice from the
+                                   grid extent (oclsph_kernel.h:421-530,
+                                   oclsph.hpp:817-819) */
+#define LF_MC_FIELD_DENSITY 1   /* SPH density field rho(x) = sum m W6(|x-x_j|)
+                                   on an explicit S^3 lattice spanning the box */
+typedef struct lf_mc_params {
+  uint32_t field;            /* LF_MC_FIELD_* */
+  float isolevel;            /* Config::isolevel */
+  float resolution;          /* Config::resolution (reference field only) */
+  float particle_size;       /* ClMcConfig::particleSize = 25 (oclsph.hpp:672) */
+  float particle_influence;  /* ClMcConfig::particleInfluence = 0.5 (:673) */
+  uint32_t samples[3];       /* lattice samples per axis (density field only) */
+  uint32_t attributes;       /* 1 = also emit normals + colours */
+} lf_mc_params;
+
+/* One output triangle: the memory layout of geometry::MeshTriangle<float>
+ * (geometry.hpp:41-44): 6 x vec3 then 3 packed ARGB colours, 84 bytes. */
+typedef struct lf_mesh_triangle {
+  float v0[3], v1[3], v2[3];
+  float n0[3], n1[3], n2[3];
+  uint32_t c[3];
+} lf_mesh_triangle;
+
+/* One particle: the memory layout of fluid::Particle<size_t,float>
+ * (fluid.hpp:53-60), 48 bytes. */
+typedef struct lf_particle {
+  uint64_t id;
+  int32_t type;
+  float mass;
+  float position[3];
+  float velocity[3];
+  uint32_t colour;
+} lf_particle;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct lf_ctx lf_ctx;
+
+/* Replaces the ocl::SphSolver constructor's context/program/queue set-up
+ * (oclsph.hpp:170-189) and the check_size layout probe (oclsph.hpp:217-243,
+ * oclsph_kernel.h:276-278): struct layouts are fixed at compile time. */
+int lf_ctx_create(int device, lf_ctx** out);
+int lf_ctx_destroy(lf_ctx* ctx);
+const char* lf_last_error(const lf_ctx* ctx);
+int lf_sync(lf_ctx* ctx);
+
+/* ---- particle state (device resident) ----------------------------------- */
+/* Upload n particles (SoA, vec3 packed as 3 floats). Replaces the SoA pack and
+ * upload of oclsph.hpp:822-853. The upload order is the "caller order". */
+int lf_state_upload(lf_ctx* ctx, size_t n, const float* position,
+                    const float* velocity, const float* mass,
+                    const int32_t* type, const uint32_t* colour);
+/* Download the current state in its current device order (after a step: the
+ * Morton order of that step, as oclsph.hpp:874-892 leaves xs). Any pointer
+ * may be NULL to skip that array. order[i] = upload index of particle i. */
+int lf_state_download(lf_ctx* ctx, size_t n, float* position, float* velocity,
+                      float* mass, int32_t* type, uint32_t* colour,
+                      uint32_t* order);
+int lf_state_size(const lf_ctx* ctx, size_t* n);
+int lf_set_wells(lf_ctx* ctx, size_t n, const lf_well* wells);
+
+/* ---- the PBF step, stage by stage --------------------------------------- */
+/* Predict (advectAndCopy, oclsph.hpp:289-325), Morton key
+ * (computeBoundAndZindex, oclsph.hpp:327-387), stable radix sort (replaces the
+ * std::sort of oclsph.hpp:733-743), payload reorder and the dense cell-start
+ * table (oclsph.hpp:746-767). */
+int lf_grid_build(lf_ctx* ctx, const lf_pbf_params* p);
+/* sph_diffuse (oclsph_kernel.h:281-303). */
+int lf_pbf_diffuse(lf_ctx* ctx, const lf_pbf_params* p);
+/* sph_lambda (oclsph_kernel.h:305-332). */
+int lf_pbf_lambda(lf_ctx* ctx, const lf_pbf_params* p);
+/* sph_delta (oclsph_kernel.h:335-384), Jacobi: reads p* and writes a second
+ * buffer, instead of the reference's in-place update. */
+int lf_pbf_delta(lf_ctx* ctx, const lf_pbf_params* p);
+/* sph_finalise (oclsph_kernel.h:404-416). */
+int lf_pbf_finalise(lf_ctx* ctx, const lf_pbf_params* p);
+/* XSPH viscosity (absent in the reference; c = p->xsph_c, no-op when 0). */
+int lf_pbf_xsph(lf_ctx* ctx, const lf_pbf_params* p);
+/* nsteps whole steps: grid, [diffuse], iterations x (lambda, delta),
+ * finalise, xsph (runSphKernel, oclsph.hpp:563-641, plus the host pre-pass). */
+int lf_pbf_step(lf_ctx* ctx, const lf_pbf_params* p, uint32_t nsteps);
+
+/* ---- marching cubes ------------------------------------------------------ */
+/* Field sampling: mc_lattice (oclsph_kernel.h:421-530) for the reference
+ * field, or the SPH density field. Needs lf_grid_build for the reference
+ * field (cell table) and a finished step (final positions). */
+int lf_mc_sample(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m);
+/* Case index + triangle count per cube and the device-side exclusive scan
+ * of the counts (mc_size, oclsph_kernel.h:545-596, plus the host sum of
+ * oclsph.hpp:461-465, done on the device). */
+int lf_mc_classify_count(lf_ctx* ctx, const lf_mc_params* m);
+/* Triangle emission at scanned offsets, in cube-index order (mc_eval,
+ * oclsph_kernel.h:632-735; deterministic instead of atomic_inc). */
+int lf_mc_emit(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m);
+/* Triangle count of the last emit (synchronises). */
+int lf_mc_count(lf_ctx* ctx, size_t* ntri);
+/* Copy up to cap triangles out (synchronises). */
+int lf_mc_download(lf_ctx* ctx, lf_mesh_triangle* out, size_t cap,
+                   size_t* ntri);
+/* Maximum triangles kept on the device (default 16M); LF_ERR_CAPACITY is
+ * reported by lf_mc_count / lf_mc_download when an emit overflowed it. */
+int lf_mc_set_capacity(lf_ctx* ctx, size_t max_triangles);
+
+/* ---- the drop-in step ----------------------------------------------------- */
+/* The core of ocl::SphSolver::advance (oclsph.hpp:661-936) after sources and
+ * drains: upload, one step, marching cubes on the reference field, readback.
+ * xs is permuted into this step's Morton order and overwritten as in
+ * oclsph.hpp:874-892. *ntri receives the triangle count; at most tri_cap
+ * triangles are written to tris (tris may be NULL when tri_cap is 0). */
+int lf_advance(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m,
+               lf_particle* xs, size_t n, lf_mesh_triangle* tris,
+               size_t tri_cap, size_t* ntri);
+
+/* ---- inspection (parity tests, teacher forcing) ------------------------- */
+#define LF_ARR_KEYS 1       /* u32[n]   sorted Morton keys */
+#define LF_ARR_ORDER 2      /* u32[n]   upload index of each sorted particle */
+#define LF_ARR_TABLE 3      /* u32[G]   dense cell-start table */
+#define LF_ARR_PSTAR 4      /* f32[n*4] p* (x,y,z,mass) in sorted order */
+#define LF_ARR_LAMBDA 5     /* f32[n*4] (p*, lambda) of the last lambda pass */
+#define LF_ARR_POSITION 6   /* f32[n*4] world position (x,y,z,mass) */
+#define LF_ARR_VELOCITY 7   /* f32[n*4] velocity (x,y,z,colour bits) */
+#define LF_ARR_TYPE 8       /* u8[n]    particle type */
+#define LF_ARR_DIFFUSED 9   /* u32[n]   diffused ARGB colour */
+#define LF_ARR_FIELD 10     /* f32[S]   MC field values (x-major lattice) */
+#define LF_ARR_MCNORMAL 11  /* f32[S*4] MC field normal (x,y,z,0) */
+#define LF_ARR_MCCOLOUR 12  /* u32[S]   MC field colour (ARGB bytes) */
+#define LF_ARR_CASES 13     /* u8[M]    MC cube case index */
+#define LF_ARR_DENSITY 14   /* f32[n]   density of the last lambda pass */
+int lf_read(lf_ctx* ctx, int array, void* dst, size_t bytes);
+/* Keep the case index of every cube for lf_read(LF_ARR_CASES) (off by
+ * default: 1 byte per cube of extra traffic). */
+int lf_mc_keep_cases(lf_ctx* ctx, int on);
+int lf_write(lf_ctx* ctx, int array, const void* src, size_t bytes);
+/* Grid geometry of the last lf_grid_build: G, extent[3], min_extent[3]. */
+int lf_grid_info(const lf_ctx* ctx, uint32_t* table_size, uint32_t extent[3],
+                 float min_extent[3]);
+/* MC lattice geometry of the last lf_mc_sample. */
+int lf_mc_info(const lf_ctx* ctx, uint32_t samples[3], float origin[3],
+               float step[3]);
+
+/* ---- timing ----------------------------------------------------------------
+ * With profiling on, every kernel launch of the context is bracketed by HIP
+ * events on the context's stream; lf_kernel_stats reports per kernel name the
+ * launch count and summed device time. */
+int lf_profile_enable(lf_ctx* ctx, int on);
+int lf_profile_reset(lf_ctx* ctx);
+/* Writes "name count total_ms\n" lines into buf (NUL terminated). */
+int lf_kernel_stats(lf_ctx* ctx, char* buf, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBFLUID_AMD_H */

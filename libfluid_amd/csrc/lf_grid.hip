@@ -1,0 +1,157 @@
+// Neighbour-search grid on gfx950: predict + Morton key, stable radix sort
+// (lf_sort.hip), payload reorder and the dense cell-start table.
+//
+// Replaces the host pre-pass of ocl::SphSolver::advance: advectAndCopy
+// (oclsph.hpp:289-325), computeBoundAndZindex (:327-387), the sort (:733-743)
+// and the grid table (:746-767), so the whole step stays on the device.
+#include "lf_internal.hpp"
+
+namespace lf {
+namespace {
+
+constexpr int kThreads = 256;
+
+// K1: p* and its Morton cell key per particle (state order).
+__global__ __launch_bounds__(kThreads) void k_predict_key(
+    uint32_t n, PbfConsts c, const float4* __restrict__ X, const float4* __restrict__ V,
+    const uint8_t* __restrict__ T, const lf_well* __restrict__ wells, int nWells,
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  float px, py, pz;
+  predictPstar(c, X[i], V[i], T[i], wells, nWells, px, py, pz);
+  keys[i] = cellKey(c, px, py, pz);
+  vals[i] = i;
+}
+
+// K3: gather the state into sorted order; p* is recomputed from the gathered
+// x, v (same arithmetic as K1, so identical bits) instead of being stored by
+// K1 and gathered. v stays the pre-gravity velocity: the reference hands the
+// device the velocity copied before the gravity update (oclsph.hpp:300,831).
+__global__ __launch_bounds__(kThreads) void k_reorder(
+    uint32_t n, PbfConsts c, const uint32_t* __restrict__ perm, const float4* __restrict__ Xin,
+    const float4* __restrict__ Vin, const uint8_t* __restrict__ Tin,
+    const uint32_t* __restrict__ IDin, const lf_well* __restrict__ wells, int nWells,
+    float4* __restrict__ Xout, float4* __restrict__ Vout, uint8_t* __restrict__ Tout,
+    uint32_t* __restrict__ IDout, float4* __restrict__ P) {
+  const uint32_t s = blockIdx.x * kThreads + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t i = min(perm[s], n - 1u);  // perm is a permutation; the clamp only bounds reads
+  const float4 x = Xin[i];
+  const float4 v = Vin[i];
+  const uint8_t t = Tin[i];
+  float px, py, pz;
+  predictPstar(c, x, v, t, wells, nWells, px, py, pz);
+  Xout[s] = x;
+  Vout[s] = v;
+  Tout[s] = t;
+  IDout[s] = IDin[i];
+  P[s] = make_float4(px, py, pz, x.w);
+}
+
+// Lower bound of key k in sorted keys[0, n).
+__device__ inline uint32_t lowerBound(const uint32_t* __restrict__ keys, uint32_t n, uint32_t k) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+constexpr uint32_t kCells = 4096;            // cells per workgroup
+constexpr uint32_t kPerThread = kCells / kThreads;
+
+// K4: table[c] = first sorted index whose key >= c, for c in [0, G): the
+// monotone scan of oclsph.hpp:758-765 as a per-tile lower bound. Each
+// workgroup owns kCells cells: segment starts of its key range are marked in
+// LDS, then a suffix-min fills empty cells with the next start.
+__global__ __launch_bounds__(kThreads) void k_cell_table(const uint32_t* __restrict__ keys,
+                                                         uint32_t n, uint32_t G,
+                                                         uint32_t* __restrict__ table) {
+  __shared__ uint32_t local[kCells];
+  __shared__ uint32_t range[2];
+  __shared__ uint32_t thrMin[kThreads];
+  const uint32_t c0 = blockIdx.x * kCells;
+  const uint32_t c1 = min(c0 + kCells, G);
+  if (threadIdx.x < 2) range[threadIdx.x] = lowerBound(keys, n, threadIdx.x == 0 ? c0 : c1);
+  __syncthreads();
+  const uint32_t s0 = range[0], s1 = range[1];
+  for (uint32_t i = threadIdx.x; i < kCells; i += kThreads) local[i] = s1;
+  __syncthreads();
+  for (uint32_t s = s0 + threadIdx.x; s < s1; s += kThreads) {
+    const uint32_t k = keys[s];
+    if ((s == 0 || keys[s - 1] != k) && k - c0 < kCells) local[k - c0] = s;
+  }
+  __syncthreads();
+  // suffix minimum over local[0, kCells): thread t owns [t*16, t*16+16)
+  const uint32_t lo = threadIdx.x * kPerThread;
+  uint32_t v[kPerThread];
+  uint32_t m = s1;
+#pragma unroll
+  for (int j = int(kPerThread) - 1; j >= 0; --j) {
+    m = min(m, local[lo + j]);
+    v[j] = m;
+  }
+  thrMin[threadIdx.x] = m;
+  __syncthreads();
+  // suffix min of the thread minima (simple: each thread scans those after it)
+  uint32_t after = s1;
+  for (uint32_t t = threadIdx.x + 1; t < kThreads; ++t) after = min(after, thrMin[t]);
+#pragma unroll
+  for (uint32_t j = 0; j < kPerThread; ++j) {
+    const uint32_t cell = c0 + lo + j;
+    if (cell < c1) table[cell] = min(v[j], after);
+  }
+}
+
+}  // namespace
+
+int gridBuild(Ctx* c) {
+  const size_t n = c->n;
+  const PbfConsts& k = c->consts;
+  int rc;
+  const int nxt = 1 - c->cur;
+  if ((rc = ensure(c, c->sortKey[0], n, "sort keys"))) return rc;
+  if ((rc = ensure(c, c->sortKey[1], n, "sort keys"))) return rc;
+  if ((rc = ensure(c, c->sortVal[0], n, "sort values"))) return rc;
+  if ((rc = ensure(c, c->sortVal[1], n, "sort values"))) return rc;
+  if ((rc = ensure(c, c->X[nxt], n, "positions"))) return rc;
+  if ((rc = ensure(c, c->V[nxt], n, "velocities"))) return rc;
+  if ((rc = ensure(c, c->T[nxt], n, "types"))) return rc;
+  if ((rc = ensure(c, c->ID[nxt], n, "ids"))) return rc;
+  if ((rc = ensure(c, c->P, n, "p*"))) return rc;
+  if ((rc = ensure(c, c->PL, n, "p*+lambda"))) return rc;
+  if ((rc = ensure(c, c->P2, n, "p* spare"))) return rc;
+  if ((rc = ensure(c, c->rho, n, "density"))) return rc;
+  if ((rc = ensure(c, c->diffused, n, "diffused"))) return rc;
+  if ((rc = ensure(c, c->table, size_t(k.G), "cell table"))) return rc;
+  if (n == 0) return LF_OK;
+  const unsigned nb = blocksFor(n, kThreads);
+  rc = launch(c, "predict_key", k_predict_key, dim3(nb), dim3(kThreads), 0, uint32_t(n), k,
+              (const float4*)c->X[c->cur].p, (const float4*)c->V[c->cur].p,
+              (const uint8_t*)c->T[c->cur].p, (const lf_well*)c->wells.p, c->nWells,
+              c->sortKey[0].p, c->sortVal[0].p);
+  if (rc) return rc;
+  int idx = 0;
+  if ((rc = radixSort(c, n, idx))) return rc;
+  rc = launch(c, "reorder", k_reorder, dim3(nb), dim3(kThreads), 0, uint32_t(n), k,
+              (const uint32_t*)c->sortVal[idx].p, (const float4*)c->X[c->cur].p,
+              (const float4*)c->V[c->cur].p, (const uint8_t*)c->T[c->cur].p,
+              (const uint32_t*)c->ID[c->cur].p, (const lf_well*)c->wells.p, c->nWells,
+              c->X[nxt].p, c->V[nxt].p, c->T[nxt].p, c->ID[nxt].p, c->P.p);
+  if (rc) return rc;
+  rc = launch(c, "cell_table", k_cell_table, dim3(blocksFor(k.G, kCells)), dim3(kThreads), 0,
+              (const uint32_t*)c->sortKey[idx].p, uint32_t(n), k.G, c->table.p);
+  if (rc) return rc;
+  c->keysIdx = idx;
+  c->gridG = k.G;
+  c->cur = nxt;
+  c->gridValid = true;
+  return LF_OK;
+}
+
+}  // namespace lf

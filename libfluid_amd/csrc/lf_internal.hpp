@@ -1,0 +1,289 @@
+// Internal declarations of the gfx950 PBF + marching-cubes library: the
+// context, device-side math shared by the kernels, and the host-side launchers
+// each .hip file provides. Paths cited are relative to the libfluid reference
+// root (tom91136/libfluid).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "libfluid_amd.h"
+
+// ---------------------------------------------------------------------------
+// Arithmetic contract (identical op order in every kernel so the FP32 path is
+// reproducible bit for bit against the CPU restatement):
+//  * the library is compiled with -ffp-contract=off; each fused multiply-add
+//    is an explicit __builtin_fmaf;
+//  * sqrt and division are IEEE correctly rounded (hipcc's default
+//    -fhip-fp32-correctly-rounded-divide-sqrt);
+//  * d2 = fma(dz,dz, fma(dy,dy, dx*dx)) with d = p_a - p_b;
+//  * W6(r) = poly6 * t*t*t, t = fma(-r, r, h*h), for r <= h
+//    gradW  = d * (spiky * ((h-r)*(h-r) / r)) for 1e-8 <= r <= h
+//    (oclsph_kernel.h:50-71);
+//  * "r <= h" is tested on d2 against rmax2, the largest float whose
+//    correctly rounded square root is <= h, which is the same predicate
+//    without the square root.
+// ---------------------------------------------------------------------------
+
+namespace lf {
+
+// Constants folded on the host from lf_pbf_params (FP32, the reference's
+// folding order of oclsph_kernel.h:20-51).
+struct PbfConsts {
+  float h, hh, rmax2, poly6, spiky, rhoRecip, rho0, cfm, corrK, wdq, invDt, dt, scale, vd,
+      xsph;
+  float minExt[3];   // minBound/scale - 2h (oclsph.hpp:359)
+  float minB[3], maxB[3];
+  float grav[3];
+  uint32_t G;        // dense cell-table size Morton(extent) (oclsph.hpp:749)
+  uint32_t extent[3];
+  uint32_t flags;
+};
+
+int foldConsts(const lf_pbf_params& p, PbfConsts& c, std::string& err);
+
+// ---- Morton codes (curves.h:56-72), 10 bits per axis ----------------------
+__host__ __device__ inline uint32_t spread10(uint32_t v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__host__ __device__ inline uint32_t mortonEncode(uint32_t x, uint32_t y, uint32_t z) {
+  return spread10(x) | (spread10(y) << 1) | (spread10(z) << 2);
+}
+constexpr uint32_t kMortonX = 0x09249249u;  // x bits of a 30-bit code
+constexpr uint32_t kMortonY = kMortonX << 1;
+constexpr uint32_t kMortonZ = kMortonX << 2;
+
+// Dilated-integer step of one axis inside a Morton code: the axis bits of
+// code+/-1 along that axis, wrapping mod 1024 like the reference's size_t
+// arithmetic masked to 10 bits (curves.h:56-72).
+__device__ inline uint32_t dilatedInc(uint32_t code, uint32_t mask) {
+  return ((code | ~mask) + 1u) & mask;
+}
+__device__ inline uint32_t dilatedDec(uint32_t code, uint32_t mask) {
+  return ((code & mask) - 1u) & mask;
+}
+
+// static_cast<size_t>((p - minExt)/h) of oclsph.hpp:327-332 masked to 10 bits;
+// negative or NaN offsets (undefined in the reference) map to 0.
+__device__ inline uint32_t cellCoord(float q) {
+  if (!(q >= 0.0f)) return 0u;
+  if (q >= 4294967296.0f) return 1023u;
+  return static_cast<uint32_t>(q) & 1023u;
+}
+
+__device__ inline uint32_t cellKey(const PbfConsts& c, float px, float py, float pz) {
+  const uint32_t qx = cellCoord((px - c.minExt[0]) / c.h);
+  const uint32_t qy = cellCoord((py - c.minExt[1]) / c.h);
+  const uint32_t qz = cellCoord((pz - c.minExt[2]) / c.h);
+  return mortonEncode(qx, qy, qz);
+}
+
+__device__ inline float dist2(float dx, float dy, float dz) {
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+__device__ inline float poly6W(const PbfConsts& c, float r) {
+  const float t = __builtin_fmaf(-r, r, c.hh);
+  return c.poly6 * ((t * t) * t);
+}
+// caller guarantees r <= h
+__device__ inline float spikyScale(const PbfConsts& c, float r) {
+  if (!(r >= 1e-8f)) return 0.0f;
+  const float q = c.h - r;
+  return c.spiky * ((q * q) / r);
+}
+
+// 27-cell visit in the reference order (oclsph_kernel.h:150-192): x fastest,
+// then y, then z; a cell c holds [table[c], table[c+1]) when c+1 < G and is
+// empty otherwise (the reference's end = start, :186-187).
+template <typename Fn>
+__device__ inline void forEachNeighbourCell(uint32_t key, const uint32_t* __restrict__ table,
+                                            uint32_t G, Fn&& fn) {
+  const uint32_t xs[3] = {dilatedDec(key, kMortonX), key & kMortonX, dilatedInc(key, kMortonX)};
+  const uint32_t ys[3] = {dilatedDec(key, kMortonY), key & kMortonY, dilatedInc(key, kMortonY)};
+  const uint32_t zs[3] = {dilatedDec(key, kMortonZ), key & kMortonZ, dilatedInc(key, kMortonZ)};
+#pragma unroll
+  for (int iz = 0; iz < 3; ++iz) {
+#pragma unroll
+    for (int iy = 0; iy < 3; ++iy) {
+      const uint32_t yz = ys[iy] | zs[iz];
+#pragma unroll
+      for (int ix = 0; ix < 3; ++ix) {
+        const uint32_t code = xs[ix] | yz;
+        if (code + 1u < G) fn(table[code], table[code + 1u]);
+      }
+    }
+  }
+}
+
+// ---- predict (advectAndCopy, oclsph.hpp:289-325) ---------------------------
+__device__ inline void predictPstar(const PbfConsts& c, float4 x, float4 v, uint8_t type,
+                                    const lf_well* __restrict__ wells, int nw, float& px,
+                                    float& py, float& pz) {
+  if (type == LF_OBSTACLE) {
+    px = x.x / c.scale;
+    py = x.y / c.scale;
+    pz = x.z / c.scale;
+    return;
+  }
+  const float m = x.w;
+  float f[3] = {m * c.grav[0], m * c.grav[1], m * c.grav[2]};
+  for (int i = 0; i < nw; ++i) {  // oclsph.hpp:307-317
+    const float d[3] = {wells[i].centre[0] - x.x, wells[i].centre[1] - x.y,
+                        wells[i].centre[2] - x.z};
+    const float dist = __builtin_sqrtf(dist2(d[0], d[1], d[2]));
+    if (dist < 75.0f) {
+      const float inv = 1.0f / dist;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float fk = ((d[k] * inv) * wells[i].force) * m;
+        fk = fk / (dist * dist);
+        fk = fminf(fmaxf(fk, -10.0f), 10.0f);
+        f[k] += fk;
+      }
+    }
+  }
+  const float vx = __builtin_fmaf(f[0], c.dt, v.x);
+  const float vy = __builtin_fmaf(f[1], c.dt, v.y);
+  const float vz = __builtin_fmaf(f[2], c.dt, v.z);
+  px = __builtin_fmaf(vx, c.dt, x.x / c.scale);
+  py = __builtin_fmaf(vy, c.dt, x.y / c.scale);
+  pz = __builtin_fmaf(vz, c.dt, x.z / c.scale);
+}
+
+// ---- context ----------------------------------------------------------------
+template <typename T> struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;  // elements
+};
+
+struct KernelStat {
+  uint64_t count = 0;
+  double ms = 0.0;
+};
+
+struct PendingTiming {
+  const char* name;
+  hipEvent_t start, stop;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  size_t n = 0;
+  int cur = 0;  // which state buffer holds the current state
+  DevBuf<float4> X[2], V[2];  // world position + mass; velocity + colour bits
+  DevBuf<uint8_t> T[2];       // particle type
+  DevBuf<uint32_t> ID[2];     // upload index
+  DevBuf<uint32_t> sortKey[2], sortVal[2];
+  DevBuf<float4> P, PL, P2;   // p* (+mass), p* + lambda, spare
+  DevBuf<float> rho;          // density of the last lambda pass
+  DevBuf<uint32_t> diffused;
+  DevBuf<uint32_t> table;
+  DevBuf<uint32_t> sortHist;
+  DevBuf<lf_well> wells;
+  int nWells = 0;
+  std::vector<lf_well> hostWells;
+
+  bool gridValid = false;
+  bool stepDone = false;
+  PbfConsts consts{};
+  lf_pbf_params params{};
+  uint32_t gridG = 0;  // table size of the last grid build
+  int keysIdx = 0;  // which sortKey/sortVal holds the sorted result
+
+  // marching cubes
+  DevBuf<float> field;
+  DevBuf<float4> mcNormal;
+  DevBuf<uint32_t> mcColour;
+  DevBuf<uint8_t> mcCases;
+  DevBuf<uint32_t> mcBlock;   // per-block triangle counts -> offsets
+  DevBuf<uint32_t> mcTotal;   // [0] = total triangles of the last emit
+  DevBuf<float> triPos;       // 9 floats per triangle
+  DevBuf<float> triNrm;       // 9 floats per triangle (attributes)
+  DevBuf<uint32_t> triCol;    // 3 per triangle (attributes)
+  size_t triCap = size_t(16) << 20;
+  uint32_t S[3] = {0, 0, 0};
+  float mcOrigin[3] = {0, 0, 0}, mcStep[3] = {0, 0, 0};
+  bool mcSampled = false, mcCounted = false, mcEmitted = false;
+  int mcAttr = 0;
+  bool keepCases = false;
+
+  // profiling
+  bool prof = false;
+  std::vector<PendingTiming> pending;
+  std::vector<hipEvent_t> eventPool;
+  std::map<std::string, KernelStat> stats;
+};
+
+// Error helpers: record text, return status.
+int fail(Ctx* c, int code, const std::string& what);
+int hipFail(Ctx* c, hipError_t e, const char* where);
+#define LF_HIP(ctx, expr)                                      \
+  do {                                                         \
+    hipError_t _e = (expr);                                    \
+    if (_e != hipSuccess) return ::lf::hipFail((ctx), _e, #expr); \
+  } while (0)
+
+template <typename T> int ensure(Ctx* c, DevBuf<T>& b, size_t count, const char* what) {
+  if (count <= b.cap) return LF_OK;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    if (e != hipSuccess) return hipFail(c, e, what);
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t want = count + count / 4 + 256;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&b.p), want * sizeof(T));
+  if (e != hipSuccess) return hipFail(c, e, what);
+  b.cap = want;
+  return LF_OK;
+}
+template <typename T> void release(DevBuf<T>& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+// Kernel timing brackets (profiling only).
+void timingBegin(Ctx* c, const char* name, PendingTiming& t);
+void timingEnd(Ctx* c, PendingTiming& t);
+
+template <typename... KArgs, typename... Args>
+int launch(Ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
+           size_t shmem, Args... args) {
+  PendingTiming t{};
+  if (c->prof) timingBegin(c, name, t);
+  hipLaunchKernelGGL(kernel, grid, block, shmem, c->stream, args...);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hipFail(c, e, name);
+  if (c->prof) timingEnd(c, t);
+  return LF_OK;
+}
+
+inline unsigned blocksFor(size_t n, unsigned per) { return unsigned((n + per - 1) / per); }
+
+// ---- launchers (one .hip file each) ----------------------------------------
+int gridBuild(Ctx* c);                        // lf_grid.hip
+int radixSort(Ctx* c, size_t n, int& outIdx);  // lf_sort.hip
+int pbfLambda(Ctx* c);                        // lf_pbf.hip
+int pbfDelta(Ctx* c);
+int pbfFinalise(Ctx* c);
+int pbfXsph(Ctx* c);
+int pbfDiffuse(Ctx* c);
+int mcSample(Ctx* c, const lf_mc_params& m);   // lf_mc.hip
+int mcClassifyCount(Ctx* c, const lf_mc_params& m);
+int mcEmit(Ctx* c, const lf_mc_params& m);
+
+}  // namespace lf

@@ -1,0 +1,496 @@
+// Marching cubes on gfx950: field sampling, case classification + triangle
+// count, a device-side scan of the counts, and triangle emission at the
+// scanned offsets (cube-index order, so the output is deterministic; the
+// reference appends with atomic_inc, oclsph_kernel.h:713).
+//
+// Lattice layout: sample (x,y,z) at index (x*Sy + y)*Sz + z (index3d,
+// curves.h:17-20); cube (x,y,z) at (x*My + y)*Mz + z with M = S - 1 (to3d,
+// curves.h:22-27). Corner order: CUBE_OFFSETS of oclsph_kernel.h:533-542.
+#include "fluid/mctable.hpp"
+#include "lf_internal.hpp"
+
+namespace lf {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kCubesPerThread = 4;
+constexpr uint32_t kCubesPerBlock = kThreads * kCubesPerThread;
+
+struct McTables {
+  uint16_t edge[256];
+  uint8_t ntri[256];
+  uint8_t tri[256][16];  // edge indices, 255 terminated
+};
+constexpr McTables makeTables() {
+  McTables t{};
+  for (int ci = 0; ci < 256; ++ci) {
+    t.edge[ci] = uint16_t(mctable::EdgeTable[ci]);
+    t.ntri[ci] = uint8_t(mctable::EdgeTable[ci] == 0 ? 0 : mctable::NumVertsTable[ci] / 3);
+    for (int i = 0; i < 16; ++i)
+      t.tri[ci][i] = uint8_t(mctable::TriTable[ci][i] < 0 ? 255 : mctable::TriTable[ci][i]);
+  }
+  return t;
+}
+__constant__ McTables cTab = makeTables();
+
+__constant__ int cCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                  {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+__constant__ int cEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                 {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+struct Lattice {
+  uint32_t S[3];
+  float origin[3];  // solver units
+  float step[3];
+};
+
+// ---- field sampling -----------------------------------------------------------
+
+// SPH density field rho(x) = sum over fluid b of m_b W6(|x - p*_b|); P.w is the
+// fluid mass (0 for obstacles) after finalise.
+__global__ __launch_bounds__(kThreads) void k_mc_density(uint64_t total, Lattice L, PbfConsts c,
+                                                         const uint32_t* __restrict__ table,
+                                                         const float4* __restrict__ P,
+                                                         float* __restrict__ field) {
+  const uint64_t idx = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (idx >= total) return;
+  const uint32_t z = uint32_t(idx % L.S[2]);
+  const uint64_t xy = idx / L.S[2];
+  const uint32_t y = uint32_t(xy % L.S[1]);
+  const uint32_t x = uint32_t(xy / L.S[1]);
+  const float px = __builtin_fmaf(float(x), L.step[0], L.origin[0]);
+  const float py = __builtin_fmaf(float(y), L.step[1], L.origin[1]);
+  const float pz = __builtin_fmaf(float(z), L.step[2], L.origin[2]);
+  float rho = 0.0f;
+  forEachNeighbourCell(cellKey(c, px, py, pz), table, c.G, [&](uint32_t s, uint32_t e) {
+    for (uint32_t b = s; b < e; ++b) {
+      const float4 pb = P[b];
+      const float d2 = dist2(px - pb.x, py - pb.y, pz - pb.z);
+      if (d2 <= c.rmax2) rho = __builtin_fmaf(pb.w, poly6W(c, __builtin_sqrtf(d2)), rho);
+    }
+  });
+  field[idx] = rho;
+}
+
+// mc_lattice (oclsph_kernel.h:421-530): metaball field sum of size/|l|^infl,
+// normal, mean colour; 27 cells clamped to the extent. Undefined reference
+// cases are written as zeros (the skipped last z-curve cell, colour/N at N=0).
+__global__ __launch_bounds__(kThreads) void k_mc_reference(
+    uint64_t total, Lattice L, PbfConsts c, float res, float psize, float infl,
+    const uint32_t* __restrict__ table, const uint8_t* __restrict__ T,
+    const float4* __restrict__ X, const float4* __restrict__ V, float* __restrict__ field,
+    float4* __restrict__ normalOut, uint32_t* __restrict__ colourOut) {
+  const uint64_t idx = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (idx >= total) return;
+  const uint32_t z = uint32_t(idx % L.S[2]);
+  const uint64_t xy = idx / L.S[2];
+  const uint32_t y = uint32_t(xy % L.S[1]);
+  const uint32_t x = uint32_t(xy / L.S[1]);
+  const float step = c.h / res;
+  const float ax = __builtin_fmaf(float(x), step, c.minExt[0]) * c.scale;
+  const float ay = __builtin_fmaf(float(y), step, c.minExt[1]) * c.scale;
+  const float az = __builtin_fmaf(float(z), step, c.minExt[2]) * c.scale;
+  const uint32_t cx = cellCoord(float(x) / res), cy = cellCoord(float(y) / res),
+                 cz = cellCoord(float(z) / res);
+  field[idx] = 0.0f;
+  normalOut[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  colourOut[idx] = 0u;
+  if (cx == c.extent[0] && cy == c.extent[1] && cz == c.extent[2]) return;
+  const float thr = c.h * c.scale * 1.0f;
+  const float k = (-infl) * psize;
+  auto clampc = [](int v, uint32_t e) { return uint32_t(min(max(v, 0), int(e) - 1)); };
+  const uint32_t xs[3] = {clampc(int(cx) - 1, c.extent[0]), cx, clampc(int(cx) + 1, c.extent[0])};
+  const uint32_t ys[3] = {clampc(int(cy) - 1, c.extent[1]), cy, clampc(int(cy) + 1, c.extent[1])};
+  const uint32_t zs[3] = {clampc(int(cz) - 1, c.extent[2]), cz, clampc(int(cz) + 1, c.extent[2])};
+  float val = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, cc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  uint32_t N = 0;
+  for (int iz = 0; iz < 3; ++iz)
+    for (int iy = 0; iy < 3; ++iy)
+      for (int ix = 0; ix < 3; ++ix) {
+        const uint32_t code = mortonEncode(xs[ix], ys[iy], zs[iz]);
+        if (code + 1u >= c.G) continue;
+        const uint32_t e = table[code + 1u];
+        for (uint32_t b = table[code]; b < e; ++b) {
+          if (T[b] == LF_OBSTACLE) continue;
+          const float4 xb = X[b];
+          const float lx = xb.x - ax, ly = xb.y - ay, lz = xb.z - az;
+          const float len = __builtin_sqrtf(dist2(lx, ly, lz));
+          if (!(len < thr)) continue;
+          const float den = (infl == 0.5f) ? __builtin_sqrtf(len) : powf(len, infl);
+          nx = __builtin_fmaf(k, lx / den, nx);
+          ny = __builtin_fmaf(k, ly / den, ny);
+          nz = __builtin_fmaf(k, lz / den, nz);
+          val += psize / den;
+          const uint32_t col = __float_as_uint(V[b].w);
+          cc[0] += float((col >> 24) & 0xFFu);
+          cc[1] += float((col >> 16) & 0xFFu);
+          cc[2] += float((col >> 8) & 0xFFu);
+          cc[3] += float(col & 0xFFu);
+          ++N;
+        }
+      }
+  const float nn = dist2(nx, ny, nz);
+  if (nn > 0.0f) {
+    const float inv = 1.0f / __builtin_sqrtf(nn);
+    nx *= inv;
+    ny *= inv;
+    nz *= inv;
+  }
+  field[idx] = val;
+  normalOut[idx] = make_float4(nx, ny, nz, 0.0f);
+  uint32_t out = 0;
+  if (N > 0) {
+    for (int q = 0; q < 4; ++q) {
+      const float m = cc[q] / float(N);
+      const uint32_t u = !(m > 0.0f) ? 0u : (m >= 255.0f ? 255u : uint32_t(m));
+      out = (out << 8) | u;
+    }
+  }
+  colourOut[idx] = out;
+}
+
+// ---- classification -----------------------------------------------------------
+
+struct CubeGrid {
+  uint32_t S[3];
+  uint32_t M[3];
+  uint64_t cubes;
+};
+
+__device__ inline uint32_t cubeCase(const CubeGrid& g, const float* __restrict__ field,
+                                    uint64_t m, float iso, uint32_t& x, uint32_t& y,
+                                    uint32_t& z) {
+  z = uint32_t(m % g.M[2]);
+  const uint64_t xy = m / g.M[2];
+  y = uint32_t(xy % g.M[1]);
+  x = uint32_t(xy / g.M[1]);
+  const uint64_t sz = g.S[2], syz = uint64_t(g.S[1]) * g.S[2];
+  const uint64_t b = uint64_t(x) * syz + uint64_t(y) * sz + z;
+  uint32_t ci = 0;
+  const float v0 = field[b], v1 = field[b + syz], v2 = field[b + syz + sz], v3 = field[b + sz];
+  const float v4 = field[b + 1], v5 = field[b + syz + 1], v6 = field[b + syz + sz + 1],
+              v7 = field[b + sz + 1];
+  ci |= (v0 < iso) ? 1u : 0u;
+  ci |= (v1 < iso) ? 2u : 0u;
+  ci |= (v2 < iso) ? 4u : 0u;
+  ci |= (v3 < iso) ? 8u : 0u;
+  ci |= (v4 < iso) ? 16u : 0u;
+  ci |= (v5 < iso) ? 32u : 0u;
+  ci |= (v6 < iso) ? 64u : 0u;
+  ci |= (v7 < iso) ? 128u : 0u;
+  return ci;
+}
+
+// mc_size (oclsph_kernel.h:545-596): triangles per block of kCubesPerBlock
+// cubes. Cube m = block*1024 + j*256 + thread (j = 0..3).
+__global__ __launch_bounds__(kThreads) void k_mc_count(CubeGrid g, const float* __restrict__ field,
+                                                       float iso, uint32_t* __restrict__ blockCount,
+                                                       uint8_t* __restrict__ cases) {
+  __shared__ uint32_t sm[kThreads / 64];
+  uint32_t sum = 0;
+  const uint64_t base = uint64_t(blockIdx.x) * kCubesPerBlock;
+#pragma unroll
+  for (int j = 0; j < kCubesPerThread; ++j) {
+    const uint64_t m = base + uint64_t(j) * kThreads + threadIdx.x;
+    if (m < g.cubes) {
+      uint32_t x, y, z;
+      const uint32_t ci = cubeCase(g, field, m, iso, x, y, z);
+      if (cases) cases[m] = uint8_t(ci);
+      sum += cTab.ntri[ci];
+    }
+  }
+  // block sum
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  if (__lane_id() == 0) sm[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kThreads / 64; ++i) t += sm[i];
+    blockCount[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of the per-block counts (one workgroup of 1024), total into
+// total[0]: the host sum of oclsph.hpp:461-465, kept on the device.
+__global__ __launch_bounds__(1024) void k_mc_scan(uint32_t* __restrict__ counts, uint32_t nb,
+                                                  uint32_t* __restrict__ total) {
+  __shared__ uint32_t sm[16];
+  const uint32_t per = (nb + 1023) / 1024;
+  const uint32_t lo = threadIdx.x * per, hi = min(lo + per, nb);
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; ++i) s += counts[i];
+  uint32_t x = s;
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sm[w] = x;
+  __syncthreads();
+  uint32_t wb = 0, tot = 0;
+  for (int i = 0; i < 16; ++i) {
+    if (i < w) wb += sm[i];
+    tot += sm[i];
+  }
+  uint32_t run = wb + x - s;
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t t = counts[i];
+    counts[i] = run;
+    run += t;
+  }
+  if (threadIdx.x == 0) total[0] = tot;
+}
+
+// ---- emission -----------------------------------------------------------------
+
+struct EmitArgs {
+  CubeGrid g;
+  float iso;
+  float origin[3], step[3], scale;
+  uint64_t cap;
+  int attributes;
+};
+
+// Block-exclusive prefix of small counts (< 8) in thread order.
+__device__ inline uint32_t blockPrefixSmall(uint32_t v, uint32_t* sm, uint32_t* total) {
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+  const uint64_t lt = (uint64_t(1) << lane) - 1u;
+  uint32_t pre = 0, wsum = 0;
+#pragma unroll
+  for (int bit = 0; bit < 3; ++bit) {
+    const uint64_t bb = __ballot((v >> bit) & 1u);
+    pre += uint32_t(__popcll(bb & lt)) << bit;
+    wsum += uint32_t(__popcll(bb)) << bit;
+  }
+  if (lane == 0) sm[w] = wsum;
+  __syncthreads();
+  uint32_t wb = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kThreads / 64; ++i) {
+    if (i < w) wb += sm[i];
+    tot += sm[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return wb + pre;
+}
+
+// mc_eval (oclsph_kernel.h:599-735): edge intersections by linear
+// interpolation t = (iso - v0)/(v1 - v0), mix(p0, p1, t); triangles of
+// TriTable[ci] written at the scanned offset.
+__global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* __restrict__ field,
+                                                      const float4* __restrict__ normals,
+                                                      const uint32_t* __restrict__ colours,
+                                                      const uint32_t* __restrict__ blockOffset,
+                                                      float* __restrict__ triPos,
+                                                      float* __restrict__ triNrm,
+                                                      uint32_t* __restrict__ triCol) {
+  __shared__ uint32_t sm[kThreads / 64];
+  const uint64_t base = uint64_t(blockIdx.x) * kCubesPerBlock;
+  uint64_t run = blockOffset[blockIdx.x];
+  for (int j = 0; j < kCubesPerThread; ++j) {
+    const uint64_t m = base + uint64_t(j) * kThreads + threadIdx.x;
+    uint32_t ci = 0, x = 0, y = 0, z = 0;
+    if (m < A.g.cubes) ci = cubeCase(A.g, field, m, A.iso, x, y, z);
+    const uint32_t nt = cTab.ntri[ci];
+    uint32_t tot;
+    const uint32_t pre = blockPrefixSmall(nt, sm, &tot);
+    if (nt > 0) {
+      const uint64_t off = run + pre;
+      const uint64_t sz = A.g.S[2], syz = uint64_t(A.g.S[1]) * A.g.S[2];
+      float val[8], pos[8][3];
+      uint64_t sidx[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t o[3] = {x + uint32_t(cCorner[i][0]), y + uint32_t(cCorner[i][1]),
+                               z + uint32_t(cCorner[i][2])};
+        sidx[i] = uint64_t(o[0]) * syz + uint64_t(o[1]) * sz + o[2];
+        val[i] = field[sidx[i]];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          pos[i][k] = __builtin_fmaf(float(o[k]), A.step[k], A.origin[k]) * A.scale;
+      }
+      const uint32_t edges = cTab.edge[ci];
+      float ts[12][3];
+      float tval[12];
+#pragma unroll
+      for (int e = 0; e < 12; ++e) {
+        tval[e] = 0.0f;
+        if (edges & (1u << e)) {
+          const int a = cEdge[e][0], b = cEdge[e][1];
+          const float t = (A.iso - val[a]) / (val[b] - val[a]);
+          tval[e] = t;
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            ts[e][k] = __builtin_fmaf(pos[b][k] - pos[a][k], t, pos[a][k]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) ts[e][k] = 0.0f;
+        }
+      }
+      for (uint32_t i = 0; i < nt; ++i) {
+        const uint64_t slot = off + i;
+        if (slot >= A.cap) break;
+        float* dst = triPos + slot * 9;
+        for (int v = 0; v < 3; ++v) {
+          const int e = cTab.tri[ci][3 * i + v];
+          dst[3 * v + 0] = ts[e][0];
+          dst[3 * v + 1] = ts[e][1];
+          dst[3 * v + 2] = ts[e][2];
+          if (A.attributes) {
+            const int a = cEdge[e][0], b = cEdge[e][1];
+            const float t = tval[e];
+            float na[3] = {0.0f, 0.0f, 0.0f}, nb[3] = {0.0f, 0.0f, 0.0f};
+            float ca[4], cb[4];
+            if (normals) {
+              const float4 n0 = normals[sidx[a]], n1 = normals[sidx[b]];
+              na[0] = n0.x; na[1] = n0.y; na[2] = n0.z;
+              nb[0] = n1.x; nb[1] = n1.y; nb[2] = n1.z;
+            }
+            if (colours) {
+              const uint32_t c0 = colours[sidx[a]], c1 = colours[sidx[b]];
+              for (int q = 0; q < 4; ++q) {
+                ca[q] = float((c0 >> (24 - 8 * q)) & 0xFFu);
+                cb[q] = float((c1 >> (24 - 8 * q)) & 0xFFu);
+              }
+            } else {
+              for (int q = 0; q < 4; ++q) ca[q] = cb[q] = 255.0f;
+            }
+            float* nd = triNrm + slot * 9;
+            for (int k = 0; k < 3; ++k) nd[3 * v + k] = __builtin_fmaf(nb[k] - na[k], t, na[k]);
+            uint32_t packed = 0;
+            for (int q = 0; q < 4; ++q) {
+              const float mv = __builtin_fmaf(cb[q] - ca[q], t, ca[q]);
+              const uint32_t u = !(mv > 0.0f) ? 0u : (mv >= 255.0f ? 255u : uint32_t(mv));
+              packed = (packed << 8) | u;
+            }
+            triCol[slot * 3 + v] = packed;
+          }
+        }
+      }
+    }
+    run += tot;
+  }
+}
+
+}  // namespace
+
+int mcSample(Ctx* c, const lf_mc_params& m) {
+  if (!c->gridValid) return fail(c, LF_ERR_STATE, "lf_mc_sample: no grid (call lf_grid_build or lf_pbf_step)");
+  const PbfConsts& k = c->consts;
+  Lattice L{};
+  if (m.field == LF_MC_FIELD_REFERENCE) {
+    if (!(m.resolution > 0.0f)) return fail(c, LF_ERR_ARG, "lf_mc_sample: resolution must be > 0");
+    for (int i = 0; i < 3; ++i) {  // oclsph.hpp:817-819
+      L.S[i] = uint32_t(floorf(float(k.extent[i]) * m.resolution)) + 1u;
+      L.origin[i] = k.minExt[i];
+      L.step[i] = k.h / m.resolution;
+    }
+  } else if (m.field == LF_MC_FIELD_DENSITY) {
+    for (int i = 0; i < 3; ++i) {
+      if (m.samples[i] < 2) return fail(c, LF_ERR_ARG, "lf_mc_sample: samples must be >= 2 per axis");
+      L.S[i] = m.samples[i];
+      L.origin[i] = k.minB[i] / k.scale;
+      const float span = (k.maxB[i] - k.minB[i]) / k.scale;
+      L.step[i] = span / float(L.S[i] - 1u);
+    }
+  } else {
+    return fail(c, LF_ERR_ARG, "lf_mc_sample: unknown field");
+  }
+  const uint64_t total = uint64_t(L.S[0]) * L.S[1] * L.S[2];
+  if (total >= (uint64_t(1) << 40)) return fail(c, LF_ERR_ARG, "lf_mc_sample: lattice too large");
+  int rc = ensure(c, c->field, size_t(total), "mc field");
+  if (rc) return rc;
+  for (int i = 0; i < 3; ++i) {
+    c->S[i] = L.S[i];
+    c->mcOrigin[i] = L.origin[i];
+    c->mcStep[i] = L.step[i];
+  }
+  const unsigned nb = unsigned((total + kThreads - 1) / kThreads);
+  if (m.field == LF_MC_FIELD_REFERENCE) {
+    if ((rc = ensure(c, c->mcNormal, size_t(total), "mc normals"))) return rc;
+    if ((rc = ensure(c, c->mcColour, size_t(total), "mc colours"))) return rc;
+    rc = launch(c, "mc_reference_field", k_mc_reference, dim3(nb), dim3(kThreads), 0, total, L, k,
+                m.resolution, m.particle_size, m.particle_influence, (const uint32_t*)c->table.p,
+                (const uint8_t*)c->T[c->cur].p, (const float4*)c->X[c->cur].p,
+                (const float4*)c->V[c->cur].p, c->field.p, c->mcNormal.p, c->mcColour.p);
+    c->mcAttr = 1;
+  } else {
+    rc = launch(c, "mc_density_field", k_mc_density, dim3(nb), dim3(kThreads), 0, total, L, k,
+                (const uint32_t*)c->table.p, (const float4*)c->P.p, c->field.p);
+    c->mcAttr = 0;
+  }
+  if (rc) return rc;
+  c->mcSampled = true;
+  c->mcCounted = c->mcEmitted = false;
+  return LF_OK;
+}
+
+static CubeGrid cubeGrid(const Ctx* c) {
+  CubeGrid g{};
+  for (int i = 0; i < 3; ++i) {
+    g.S[i] = c->S[i];
+    g.M[i] = c->S[i] - 1u;
+  }
+  g.cubes = uint64_t(g.M[0]) * g.M[1] * g.M[2];
+  return g;
+}
+
+int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
+  if (!c->mcSampled) return fail(c, LF_ERR_STATE, "lf_mc_classify_count: no field (call lf_mc_sample)");
+  const CubeGrid g = cubeGrid(c);
+  const uint64_t nbl = (g.cubes + kCubesPerBlock - 1) / kCubesPerBlock;
+  int rc = ensure(c, c->mcBlock, size_t(nbl) + 1, "mc block counts");
+  if (rc) return rc;
+  if ((rc = ensure(c, c->mcTotal, 4, "mc total"))) return rc;
+  if (c->keepCases && (rc = ensure(c, c->mcCases, size_t(g.cubes), "mc cases"))) return rc;
+  if (nbl > 0) {
+    rc = launch(c, "mc_classify_count", k_mc_count, dim3(unsigned(nbl)), dim3(kThreads), 0, g,
+                (const float*)c->field.p, m.isolevel, c->mcBlock.p,
+                c->keepCases ? c->mcCases.p : (uint8_t*)nullptr);
+    if (rc) return rc;
+  }
+  rc = launch(c, "mc_scan", k_mc_scan, dim3(1), dim3(1024), 0, c->mcBlock.p, uint32_t(nbl),
+              c->mcTotal.p);
+  if (rc) return rc;
+  c->mcCounted = true;
+  return LF_OK;
+}
+
+int mcEmit(Ctx* c, const lf_mc_params& m) {
+  if (!c->mcCounted) return fail(c, LF_ERR_STATE, "lf_mc_emit: no counts (call lf_mc_classify_count)");
+  const CubeGrid g = cubeGrid(c);
+  const uint64_t nbl = (g.cubes + kCubesPerBlock - 1) / kCubesPerBlock;
+  int rc = ensure(c, c->triPos, c->triCap * 9, "triangle positions");
+  if (rc) return rc;
+  const int attr = (m.attributes != 0) ? 1 : 0;
+  if (attr) {
+    if ((rc = ensure(c, c->triNrm, c->triCap * 9, "triangle normals"))) return rc;
+    if ((rc = ensure(c, c->triCol, c->triCap * 3, "triangle colours"))) return rc;
+  }
+  EmitArgs A{};
+  A.g = g;
+  A.iso = m.isolevel;
+  for (int i = 0; i < 3; ++i) {
+    A.origin[i] = c->mcOrigin[i];
+    A.step[i] = c->mcStep[i];
+  }
+  A.scale = c->consts.scale;
+  A.cap = c->triCap;
+  A.attributes = attr;
+  if (nbl > 0) {
+    rc = launch(c, "mc_emit", k_mc_emit, dim3(unsigned(nbl)), dim3(kThreads), 0, A,
+                (const float*)c->field.p,
+                (const float4*)((attr && c->mcAttr) ? c->mcNormal.p : nullptr),
+                (const uint32_t*)((attr && c->mcAttr) ? c->mcColour.p : nullptr),
+                (const uint32_t*)c->mcBlock.p, c->triPos.p, attr ? c->triNrm.p : (float*)nullptr,
+                attr ? c->triCol.p : (uint32_t*)nullptr);
+    if (rc) return rc;
+  }
+  c->mcEmitted = true;
+  return LF_OK;
+}
+
+}  // namespace lf

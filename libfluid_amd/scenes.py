@@ -1,0 +1,146 @@
+"""Seeded dam-break scenes for parity tests and benchmarks.
+
+A scene is a jittered lattice block (the lattice generator of
+samples/main.cpp:36-56, x outermost, z innermost, ids in that order) plus the
+solver parameters. The jitter is a counter-based hash of (seed, particle id,
+axis), so any consumer regenerates bit-identical inputs.
+
+Scaling: the recipe at spacing s keeps libfluid's rest-state ratio by giving
+each particle the mass (s / 0.05)^3 (1.0 at the 8,000-particle, s = 0.05
+scene), so every scene starts at the same density relative to the rest
+density 6378 (oclsph_kernel.h:21) and h = 2 s gives ~33 lattice neighbours.
+"""
+import dataclasses
+from typing import Sequence, Tuple
+
+import numpy as np
+
+REST_DENSITY = 6378.0
+BASE_SPACING = 0.05
+
+_M1 = np.uint64(0x9E3779B97F4A7C15)
+_M2 = np.uint64(0xBF58476D1CE4E5B9)
+_M3 = np.uint64(0x94D049BB133111EB)
+
+
+def hash_uniform(seed, ids, axis):
+    """splitmix64 of (seed, id, axis) -> float64 uniform in [0, 1)."""
+    with np.errstate(over="ignore"):
+        ids = np.asarray(ids, dtype=np.uint64)
+        x = (np.uint64(seed) * _M1 + ids * _M2 + np.uint64(axis + 1) * _M3)
+        x = (x ^ (x >> np.uint64(30))) * _M2
+        x = (x ^ (x >> np.uint64(27))) * _M3
+        x = x ^ (x >> np.uint64(31))
+    return (x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def lattice_block(dims, spacing, origin, jitter, seed, id_offset=0):
+    """float32 positions of a dims[0] x dims[1] x dims[2] lattice, x outermost."""
+    nx, ny, nz = dims
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    idx = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1).astype(np.float64)
+    ids = np.arange(idx.shape[0], dtype=np.uint64) + np.uint64(id_offset)
+    pos = np.asarray(origin, np.float64) + idx * spacing
+    if jitter:
+        for k in range(3):
+            pos[:, k] += (2.0 * hash_uniform(seed, ids, k) - 1.0) * jitter
+    return pos.astype(np.float32), ids
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    h: float
+    spacing: float
+    position: np.ndarray
+    velocity: np.ndarray
+    mass: np.ndarray
+    type: np.ndarray
+    colour: np.ndarray
+    ids: np.ndarray
+    min_bound: Tuple[float, float, float]
+    max_bound: Tuple[float, float, float]
+    iterations: int = 4
+    dt: float = 1.0 / 120.0
+    gravity: Tuple[float, float, float] = (0.0, -9.8, 0.0)
+    xsph_c: float = 0.01
+    steps: int = 10
+    mc_samples: Sequence[int] = (64, 64, 64)
+    isolevel: float = 0.5 * REST_DENSITY
+    seed: int = 1
+
+    @property
+    def n(self):
+        return self.position.shape[0]
+
+    def params(self, flags=0, xsph_c=None, iterations=None):
+        from . import pbf_params
+        return pbf_params(self.h, dt=self.dt, scale=1.0,
+                          iterations=self.iterations if iterations is None else iterations,
+                          gravity=self.gravity, min_bound=self.min_bound,
+                          max_bound=self.max_bound,
+                          xsph_c=self.xsph_c if xsph_c is None else xsph_c, flags=flags)
+
+    def mc(self, attributes=0):
+        from . import mc_density_params
+        return mc_density_params(self.isolevel, tuple(self.mc_samples), attributes)
+
+
+def _scene(name, blocks, spacing, box, seed, steps, mc_samples, colour=0xFF2D1010):
+    pos, ids = [], []
+    off = 0
+    for dims, origin in blocks:
+        p, i = lattice_block(dims, spacing, origin, 0.1 * spacing, seed, off)
+        pos.append(p)
+        ids.append(i)
+        off += p.shape[0]
+    pos = np.concatenate(pos)
+    n = pos.shape[0]
+    return Scene(
+        name=name, h=2.0 * spacing, spacing=spacing, position=pos,
+        velocity=np.zeros((n, 3), np.float32),
+        mass=np.full(n, (spacing / BASE_SPACING) ** 3, np.float32),
+        type=np.zeros(n, np.int32), colour=np.full(n, colour, np.uint32),
+        ids=np.concatenate(ids), min_bound=(0.0, 0.0, 0.0), max_bound=tuple(box),
+        steps=steps, mc_samples=mc_samples, seed=seed)
+
+
+def dam_break_8k(seed=1):
+    """8,000 particles: 20^3 lattice, spacing 0.05 from (0.025,..), jitter
+    +-0.005, 2x1x1 box, h = 0.1, 10 steps, marching cubes at 64^3."""
+    s = 0.05
+    return _scene("dam_break_8k", [((20, 20, 20), (s / 2,) * 3)], s, (2.0, 1.0, 1.0), seed, 10,
+                  (64, 64, 64))
+
+
+def dam_break_1m(seed=2):
+    """1,048,576 particles: 128^3 lattice, spacing 0.0125, 4x2x2 box, h = 0.025,
+    100 steps, marching cubes at 256^3 every step."""
+    s = 0.0125
+    return _scene("dam_break_1m", [((128, 128, 128), (s / 2,) * 3)], s, (4.0, 2.0, 2.0), seed,
+                  100, (256, 256, 256))
+
+
+def double_dam_break_4m(seed=4):
+    """4,194,304 particles: two 128^3 blocks, spacing 0.00625, h = 0.0125, at
+    opposite x ends of a 4x2x2 box, 20 steps."""
+    s = 0.00625
+    side = 128 * s
+    return _scene("double_dam_break_4m",
+                  [((128, 128, 128), (s / 2,) * 3),
+                   ((128, 128, 128), (4.0 - side + s / 2, s / 2, s / 2))],
+                  s, (4.0, 2.0, 2.0), seed, 20, (256, 256, 256))
+
+
+def small_block(n_side=10, spacing=0.05, seed=7, box=(1.0, 1.0, 1.0), jitter=True):
+    """A small jittered block for quick parity checks."""
+    s = spacing
+    p, ids = lattice_block((n_side,) * 3, s, (s / 2 + 0.1,) * 3, 0.1 * s if jitter else 0.0,
+                           seed)
+    n = p.shape[0]
+    return Scene(name="small_block_%d" % n, h=2 * s, spacing=s, position=p,
+                 velocity=np.zeros((n, 3), np.float32),
+                 mass=np.full(n, (s / BASE_SPACING) ** 3, np.float32),
+                 type=np.zeros(n, np.int32), colour=np.full(n, 0xFF2D1010, np.uint32), ids=ids,
+                 min_bound=(0.0, 0.0, 0.0), max_bound=tuple(box), steps=2,
+                 mc_samples=(32, 32, 32), seed=seed)

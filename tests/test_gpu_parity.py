@@ -1,0 +1,302 @@
+"""GPU parity: the HIP path through the C ABI against the CPU restatement
+(oracle/) on identical seeded inputs.
+
+The FP32 path follows one arithmetic contract on both sides (explicit FMAs,
+correctly rounded sqrt/div, the reference's 27-cell order, stable sort), so
+the bar is bit-exact for keys, cell tables, orders, lambda, density, p*,
+positions, velocities, colours, MC fields, case indices and triangles.
+Comparisons use == on values (so -0 == +0).
+"""
+import numpy as np
+import pytest
+
+import libfluid_amd as lf
+import oracle
+from libfluid_amd import scenes
+from libfluid_amd._abi import (LF_ARR_CASES, LF_ARR_DENSITY, LF_ARR_DIFFUSED, LF_ARR_FIELD,
+                               LF_ARR_KEYS, LF_ARR_LAMBDA, LF_ARR_MCCOLOUR, LF_ARR_MCNORMAL,
+                               LF_ARR_ORDER, LF_ARR_POSITION, LF_ARR_PSTAR, LF_ARR_TABLE,
+                               LF_ARR_VELOCITY, LF_ERR_MORTON)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = lf.Context(0)
+    yield c
+    c.close()
+
+
+def _pair(ctx, sc, p, fp64=False):
+    ctx.upload(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
+    o = oracle.Pbf(p, fp64=fp64)
+    o.load(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
+    return o
+
+
+def _eq(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b).astype(a.dtype) if np.asarray(b).dtype != a.dtype else np.asarray(b)
+    if a.shape != b.shape:
+        raise AssertionError("%s: shape %s vs %s" % (what, a.shape, b.shape))
+    bad = ~((a == b) | (np.isnan(a) & np.isnan(b)) if a.dtype.kind == "f" else (a == b))
+    if bad.any():
+        idx = np.argwhere(bad)[:5]
+        raise AssertionError("%s: %d of %d differ, first %s: gpu %s oracle %s" % (
+            what, int(bad.sum()), bad.size, idx.tolist(), a[tuple(idx[0])], b[tuple(idx[0])]))
+
+
+SCENES = {
+    "8k": scenes.dam_break_8k,
+    "small": lambda: scenes.small_block(n_side=9),
+}
+
+
+@pytest.mark.parametrize("name", ["8k", "small"])
+def test_grid_bit_exact(ctx, name):
+    sc = SCENES[name]()
+    p = sc.params()
+    o = _pair(ctx, sc, p)
+    ctx.grid_build(p)
+    o.grid_build()
+    G, ext, mn = ctx.grid_info()
+    oG, oext, omn = o.grid_info()
+    assert (G, ext) == (oG, oext)
+    _eq(ctx.read(LF_ARR_KEYS), o.get(LF_ARR_KEYS), "sorted keys")
+    _eq(ctx.read(LF_ARR_ORDER), o.get(LF_ARR_ORDER), "sort order")
+    _eq(ctx.read(LF_ARR_TABLE), o.get(LF_ARR_TABLE), "cell table")
+    _eq(ctx.read(LF_ARR_PSTAR).astype(np.float64), o.get(LF_ARR_PSTAR), "p*")
+
+
+def test_grid_1m_bit_exact(ctx):
+    sc = scenes.dam_break_1m()
+    p = sc.params()
+    o = _pair(ctx, sc, p)
+    ctx.grid_build(p)
+    o.grid_build()
+    G, ext, _ = ctx.grid_info()
+    assert G == 3727478 and ext == (164, 83, 83)
+    _eq(ctx.read(LF_ARR_KEYS), o.get(LF_ARR_KEYS), "sorted keys 1M")
+    _eq(ctx.read(LF_ARR_ORDER), o.get(LF_ARR_ORDER), "sort order 1M")
+    _eq(ctx.read(LF_ARR_TABLE), o.get(LF_ARR_TABLE), "cell table 1M")
+
+
+@pytest.mark.parametrize("name", ["8k", "small"])
+@pytest.mark.parametrize("kahan", [False, True])
+def test_constraint_passes_bit_exact(ctx, name, kahan):
+    sc = SCENES[name]()
+    p = sc.params(flags=lf.LF_FLAG_KAHAN if kahan else 0)
+    o = _pair(ctx, sc, p)
+    ctx.grid_build(p)
+    o.grid_build()
+    for it in range(p.iterations):
+        ctx.lambda_pass(p)
+        o.lambda_pass()
+        _eq(ctx.read(LF_ARR_LAMBDA)[:, 3].astype(np.float64), o.get(LF_ARR_LAMBDA),
+            "lambda it%d" % it)
+        _eq(ctx.read(LF_ARR_DENSITY).astype(np.float64), o.get(LF_ARR_DENSITY), "density it%d" % it)
+        ctx.delta_pass(p)
+        o.delta_pass()
+        _eq(ctx.read(LF_ARR_PSTAR)[:, :3].astype(np.float64), o.get(LF_ARR_PSTAR)[:, :3],
+            "p* it%d" % it)
+    ctx.finalise(p)
+    o.finalise()
+    _eq(ctx.read(LF_ARR_POSITION)[:, :3].astype(np.float64), o.get(LF_ARR_POSITION), "x")
+    _eq(ctx.read(LF_ARR_VELOCITY)[:, :3].astype(np.float64), o.get(LF_ARR_VELOCITY), "v")
+    ctx.xsph(p)
+    o.xsph()
+    _eq(ctx.read(LF_ARR_VELOCITY)[:, :3].astype(np.float64), o.get(LF_ARR_VELOCITY), "v xsph")
+
+
+def test_diffuse_bit_exact(ctx):
+    sc = scenes.dam_break_8k()
+    rng = np.random.default_rng(5)
+    sc.colour = rng.integers(0, 2**32, sc.n, dtype=np.uint64).astype(np.uint32)
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
+    o = _pair(ctx, sc, p)
+    ctx.grid_build(p)
+    o.grid_build()
+    ctx.diffuse(p)
+    o.diffuse()
+    _eq(ctx.read(LF_ARR_DIFFUSED), o.get(LF_ARR_DIFFUSED), "diffused colour")
+
+
+@pytest.mark.parametrize("steps", [1, 10])
+def test_steps_bit_exact_8k(ctx, steps):
+    sc = scenes.dam_break_8k()
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
+    o = _pair(ctx, sc, p)
+    ctx.step(p, steps)
+    o.step(steps)
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position")
+    _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity")
+    _eq(g["colour"], o.get(oracle.CALLER_COLOUR), "colour")
+
+
+def test_step_1m_bit_exact(ctx):
+    sc = scenes.dam_break_1m()
+    p = sc.params()
+    o = _pair(ctx, sc, p)
+    ctx.step(p, 1)
+    o.step(1)
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position 1M")
+    _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity 1M")
+
+
+def test_density_mc_bit_exact(ctx):
+    sc = scenes.dam_break_8k()
+    p = sc.params()
+    o = _pair(ctx, sc, p)
+    ctx.step(p, 1)
+    o.step(1)
+    # the oracle's sorted state after its step equals the GPU's (checked above)
+    m = sc.mc()
+    ctx.mc_keep_cases(True)
+    ctx.mc_sample(p, m)
+    ctx.mc_classify_count(m)
+    ctx.mc_emit(p, m)
+    field = ctx.read(LF_ARR_FIELD)
+    ofield = o.mc_density(m.samples)
+    _eq(field.astype(np.float64), ofield, "density field")
+    cases = ctx.read(LF_ARR_CASES)
+    _eq(cases, oracle.mc_cases(field.astype(np.float64), m.isolevel), "case indices")
+    tris = ctx.mc_triangles()
+    origin, step = o.density_lattice(m.samples)
+    otris = oracle.mc_emit(field.astype(np.float64), m.isolevel, origin, step, 1.0)
+    assert len(tris) == len(otris) > 0
+    _eq(tris["v"], otris["v"], "triangle vertices")
+
+
+def test_reference_mc_bit_exact(ctx):
+    sc = scenes.dam_break_8k()
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
+    o = _pair(ctx, sc, p)
+    ctx.step(p, 1)
+    o.step(1)
+    m = lf.mc_reference_params(isolevel=100.0, resolution=1.0)
+    ctx.mc_sample(p, m)
+    S, ofield, onormal, ocol = o.mc_reference(m)
+    assert ctx.mc_info()[0] == S
+    _eq(ctx.read(LF_ARR_FIELD).astype(np.float64), ofield, "metaball field")
+    _eq(ctx.read(LF_ARR_MCNORMAL).astype(np.float64), onormal, "field normals")
+    _eq(ctx.read(LF_ARR_MCCOLOUR), ocol, "field colours")
+    ctx.mc_classify_count(m)
+    ctx.mc_emit(p, m)
+    tris = ctx.mc_triangles()
+    G, ext, mn = o.grid_info()
+    step = np.float32(p.h) / np.float32(m.resolution)
+    otris = oracle.mc_emit(ofield, m.isolevel, mn, (step,) * 3, 1.0, normal=onormal, colour=ocol)
+    assert len(tris) == len(otris) > 0
+    _eq(tris["v"], otris["v"], "triangle vertices")
+    _eq(tris["n"], otris["n"], "triangle normals")
+    _eq(tris["c"], otris["c"], "triangle colours")
+
+
+def test_advance_drop_in(ctx):
+    """lf_advance through the SphSolver mirror: Morton-permuted particles and
+    the reference-field mesh, against the oracle's step + mesh."""
+    sc = scenes.dam_break_8k()
+    solver = lf.SphSolver(sc.h)
+    xs = np.zeros(sc.n, dtype=lf.particle_dtype())
+    xs["id"] = sc.ids + 1000
+    xs["type"] = sc.type
+    xs["mass"] = sc.mass
+    xs["position"] = sc.position
+    xs["velocity"] = sc.velocity
+    xs["colour"] = sc.colour
+    cfg = lf.Config(dt=sc.dt, scale=1.0, resolution=1.0, isolevel=100.0, iteration=4,
+                    constant_force=sc.gravity, min_bound=sc.min_bound, max_bound=sc.max_bound)
+    out, res = solver.advance(cfg, xs)
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE, xsph_c=0.0)
+    o = oracle.Pbf(p)
+    o.load(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
+    o.step()
+    order = o.get(LF_ARR_ORDER)
+    _eq(out["id"], xs["id"][order], "ids in Morton order")
+    _eq(out["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "positions")
+    _eq(out["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocities")
+    _eq(out["colour"], o.get(oracle.CALLER_COLOUR), "colours")
+    m = lf.mc_reference_params(isolevel=100.0, resolution=1.0)
+    S, ofield, onormal, ocol = o.mc_reference(m)
+    G, ext, mn = o.grid_info()
+    step = np.float32(sc.h)
+    otris = oracle.mc_emit(ofield, 100.0, mn, (step,) * 3, 1.0, normal=onormal, colour=ocol)
+    assert len(res.triangles) == len(otris)
+    _eq(res.triangles["v"], otris["v"], "mesh vertices")
+    solver.ctx.close()
+
+
+def test_empty_and_obstacles(ctx):
+    sc = scenes.small_block(n_side=6)
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
+    # all obstacles: nothing moves
+    t = np.ones(sc.n, np.int32)
+    ctx.upload(sc.position, sc.velocity, sc.mass, t, sc.colour)
+    ctx.step(p, 2)
+    g = ctx.download()
+    assert np.array_equal(np.sort(g["position"], axis=0), np.sort(sc.position, axis=0))
+    # mixed fluid/obstacle, bit-exact against the oracle
+    t = (np.arange(sc.n) % 3 == 0).astype(np.int32)
+    ctx.upload(sc.position, sc.velocity, sc.mass, t, sc.colour)
+    o = oracle.Pbf(p)
+    o.load(sc.position, sc.velocity, sc.mass, t, sc.colour)
+    ctx.step(p, 2)
+    o.step(2)
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "mixed positions")
+    # empty input
+    ctx.upload(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0))
+    ctx.step(p, 1)
+    assert ctx.download()["position"].shape == (0, 3)
+
+
+def test_outside_bounds_and_coincident(ctx):
+    """Particles far outside the padded extent and exactly coincident pairs."""
+    sc = scenes.small_block(n_side=6)
+    pos = sc.position.copy()
+    pos[0] = (-3.0, 0.5, 0.5)      # far below minExt: cell coordinate 0
+    pos[1] = (5.0, 5.0, 5.0)       # beyond maxExt: key >= G
+    pos[3] = pos[2]                # coincident pair
+    p = sc.params()
+    ctx.upload(pos, sc.velocity, sc.mass, sc.type, sc.colour)
+    o = oracle.Pbf(p)
+    o.load(pos, sc.velocity, sc.mass, sc.type, sc.colour)
+    ctx.step(p, 1)
+    o.step(1)
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "positions")
+    assert np.isfinite(g["position"]).all()
+
+
+def test_morton_extent_error(ctx):
+    sc = scenes.small_block(n_side=4)
+    p = sc.params()
+    p.max_bound[0] = 200.0  # 2000+ cells on x: beyond the 10-bit Morton axis
+    ctx.upload(sc.position, sc.velocity, sc.mass)
+    with pytest.raises(lf.LfError) as e:
+        ctx.grid_build(p)
+    assert e.value.code == LF_ERR_MORTON
+
+
+def test_sort_properties_random_keys(ctx):
+    """Size-independent properties on a large random cloud: permutation,
+    sortedness, stability, table = lower bound."""
+    rng = np.random.default_rng(11)
+    n = 2_000_000
+    pos = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    p = lf.pbf_params(0.01, gravity=(0, 0, 0), max_bound=(1, 1, 1))
+    ctx.upload(pos, np.zeros_like(pos), np.ones(n, np.float32))
+    ctx.grid_build(p)
+    keys = ctx.read(LF_ARR_KEYS)
+    order = ctx.read(LF_ARR_ORDER)
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
+    same = np.diff(keys.astype(np.int64)) == 0
+    assert np.all(np.diff(order.astype(np.int64))[same] > 0), "sort is not stable"
+    table = ctx.read(LF_ARR_TABLE)
+    G = table.shape[0]
+    assert np.array_equal(table, np.searchsorted(keys, np.arange(G, dtype=np.uint32),
+                                                 side="left").astype(np.uint32))
