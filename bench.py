@@ -66,9 +66,10 @@ def cpu_baseline(scene, mc, budget_s=20.0):
         o.step()
         field = o.mc_density(mc.samples)
         origin, step = o.density_lattice(mc.samples)
-        oracle.mc_emit(field, mc.isolevel, origin, step, 1.0)
+        oracle.mc_emit(field, mc.isolevel, origin, step, scene.scale)
         t += time.perf_counter() - t0
         steps += 1
+        print("bench: cpu baseline step %d: %.2f s" % (steps, t), file=sys.stderr, flush=True)
         if t > budget_s / 2 and steps >= 1:
             break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -110,7 +111,7 @@ def main():
     for _ in range(args.warmup):
         one_step()
     ctx.sync()
-    ntri_warm = ctx.mc_count()
+    print("bench: warmup done", file=sys.stderr, flush=True)
 
     ctx.profile(True)
     ctx.profile_reset()
@@ -120,6 +121,7 @@ def main():
         one_step()
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    print("bench: %d timed steps in %.3f s" % (args.steps, elapsed), file=sys.stderr, flush=True)
     stats = ctx.kernel_stats()
     ntri = ctx.mc_count()
     G = ctx.grid_info()[0]
@@ -168,6 +170,10 @@ def main():
         "kernels": kernels,
         "device_ms_per_step": device_ms / args.steps,
     }
+    copy_bytes = 1 << 28
+    copy_ms = ctx.copy_probe(copy_bytes, reps=20)
+    out["copy_probe"] = {"bytes_moved": 2 * copy_bytes, "ms": copy_ms,
+                         "GBs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1)}
     ctx.close()
     if not args.no_cpu:
         cv, csteps, threads = cpu_baseline(scene, mc)

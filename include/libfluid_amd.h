@@ -63,7 +63,8 @@ typedef struct lf_pbf_params {
   float corr_k;         /* CorrK = 1e-4 (oclsph_kernel.h:30) */
   float corr_dq;        /* CorrDeltaQ / h = 0.3 (oclsph_kernel.h:26) */
   float vd;             /* VD = 0.49, velocity damping (oclsph_kernel.h:20) */
-  float xsph_c;         /* XSPH viscosity c; 0 = off (the reference has none) */
+  float xsph_c;         /* XSPH viscosity c: v += (c/rho0) sum m_j (v_j - v) W6;
+                           0 = off (the reference has none) */
   uint32_t flags;       /* LF_FLAG_* */
 } lf_pbf_params;
 
@@ -226,6 +227,10 @@ int lf_profile_enable(lf_ctx* ctx, int on);
 int lf_profile_reset(lf_ctx* ctx);
 /* Writes "name count total_ms\n" lines into buf (NUL terminated). */
 int lf_kernel_stats(lf_ctx* ctx, char* buf, size_t cap);
+/* Copy-kernel bandwidth probe (no reference counterpart: the denominator of
+ * the roofline figures): copies `bytes` device to device with a plain
+ * 16-byte-per-lane kernel ("copy_probe") reps times; *ms = mean per copy. */
+int lf_copy_probe(lf_ctx* ctx, size_t bytes, int reps, float* ms);
 
 #ifdef __cplusplus
 }

@@ -278,6 +278,12 @@ class Context:
     def profile_reset(self):
         self._check(self._lib.lf_profile_reset(self._h))
 
+    def copy_probe(self, nbytes, reps=20):
+        """Mean ms of one device-to-device copy of nbytes by a plain kernel."""
+        ms = ctypes.c_float()
+        self._check(self._lib.lf_copy_probe(self._h, nbytes, reps, ctypes.byref(ms)))
+        return ms.value
+
     def kernel_stats(self):
         """{kernel name: (launches, total ms)} over the profiled launches."""
         buf = ctypes.create_string_buffer(1 << 16)

@@ -164,6 +164,7 @@ SIGNATURES = {
     "lf_profile_enable": (_I, [_P, _I]),
     "lf_profile_reset": (_I, [_P]),
     "lf_kernel_stats": (_I, [_P, ctypes.c_char_p, _SZ]),
+    "lf_copy_probe": (_I, [_P, _SZ, _I, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None

@@ -50,6 +50,7 @@ int foldConsts(const lf_pbf_params& p, PbfConsts& c, std::string& err) {
   c.scale = p.scale;
   c.vd = p.vd;
   c.xsph = p.xsph_c;
+  c.xsphRho = c.xsph / c.rho0;
   c.flags = p.flags;
   c.wdq = hostW6(c, p.corr_dq * c.h);
   // rmax2: the largest float x with sqrtf(x) <= h
@@ -611,6 +612,12 @@ int lf_kernel_stats(lf_ctx* c, char* buf, size_t cap) {
   std::strncpy(buf, s.c_str(), cap - 1);
   buf[cap - 1] = '\0';
   return s.size() < cap ? LF_OK : LF_ERR_ARG;
+}
+
+int lf_copy_probe(lf_ctx* c, size_t bytes, int reps, float* ms) {
+  if (!c || !ms || bytes < 16 || reps < 1) return LF_ERR_ARG;
+  LF_HIP(c, hipSetDevice(c->device));
+  return lf::copyProbe(c, bytes, reps, ms);
 }
 
 /* Keep marching-cubes case indices for lf_read(LF_ARR_CASES) (parity tests). */

@@ -4,6 +4,8 @@
 // Replaces the host pre-pass of ocl::SphSolver::advance: advectAndCopy
 // (oclsph.hpp:289-325), computeBoundAndZindex (:327-387), the sort (:733-743)
 // and the grid table (:746-767), so the whole step stays on the device.
+#include <algorithm>
+
 #include "lf_internal.hpp"
 
 namespace lf {
@@ -108,7 +110,46 @@ __global__ __launch_bounds__(kThreads) void k_cell_table(const uint32_t* __restr
   }
 }
 
+// Plain streaming copy, 16 B per lane per iteration (bandwidth probe).
+__global__ __launch_bounds__(kThreads) void k_copy_probe(const float4* __restrict__ src,
+                                                         float4* __restrict__ dst, size_t n) {
+  const size_t stride = size_t(gridDim.x) * kThreads;
+  for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+int copyProbe(Ctx* c, size_t bytes, int reps, float* ms) {
+  const size_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  LF_HIP(c, hipMalloc(reinterpret_cast<void**>(&a), n * 16));
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&b), n * 16);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return hipFail(c, e, "copy probe");
+  }
+  hipEvent_t t0, t1;
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
+  int rc = LF_OK;
+  (void)hipMemsetAsync(a, 0, n * 16, c->stream);
+  const unsigned grid = unsigned(std::min<size_t>((n + kThreads - 1) / kThreads, 256 * 64));
+  rc = launch(c, "copy_probe", k_copy_probe, dim3(grid), dim3(kThreads), 0, (const float4*)a, b, n);
+  (void)hipEventRecord(t0, c->stream);
+  for (int r = 0; r < reps && rc == LF_OK; ++r)
+    rc = launch(c, "copy_probe", k_copy_probe, dim3(grid), dim3(kThreads), 0, (const float4*)a, b,
+                n);
+  (void)hipEventRecord(t1, c->stream);
+  (void)hipEventSynchronize(t1);
+  float total = 0.0f;
+  (void)hipEventElapsedTime(&total, t0, t1);
+  *ms = total / float(reps);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}
 
 int gridBuild(Ctx* c) {
   const size_t n = c->n;

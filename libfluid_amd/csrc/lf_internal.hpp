@@ -35,7 +35,7 @@ namespace lf {
 // folding order of oclsph_kernel.h:20-51).
 struct PbfConsts {
   float h, hh, rmax2, poly6, spiky, rhoRecip, rho0, cfm, corrK, wdq, invDt, dt, scale, vd,
-      xsph;
+      xsph, xsphRho;  // xsphRho = xsph / rho0
   float minExt[3];   // minBound/scale - 2h (oclsph.hpp:359)
   float minB[3], maxB[3];
   float grav[3];
@@ -285,5 +285,6 @@ int pbfDiffuse(Ctx* c);
 int mcSample(Ctx* c, const lf_mc_params& m);   // lf_mc.hip
 int mcClassifyCount(Ctx* c, const lf_mc_params& m);
 int mcEmit(Ctx* c, const lf_mc_params& m);
+int copyProbe(Ctx* c, size_t bytes, int reps, float* ms);  // lf_grid.hip
 
 }  // namespace lf

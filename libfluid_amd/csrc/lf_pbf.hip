@@ -160,8 +160,9 @@ __global__ __launch_bounds__(kThreads) void k_finalise(uint32_t n, PbfConsts c,
                      __builtin_fmaf(dZ, c.invDt, v.z) * c.vd, col);
 }
 
-// XSPH viscosity (absent in the reference): v_a += c sum_b (v_b - v_a) W6(r)
-// over fluid b within h, Jacobi (Vin -> Vout).
+// XSPH viscosity (absent in the reference): v_a += (c/rho0) sum_b m_b (v_b -
+// v_a) W6(r) over fluid b within h, Jacobi (Vin -> Vout). P.w is the fluid
+// mass after finalise.
 __global__ __launch_bounds__(kThreads) void k_xsph(uint32_t n, PbfConsts c,
                                                    const uint32_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ table,
@@ -184,16 +185,16 @@ __global__ __launch_bounds__(kThreads) void k_xsph(uint32_t n, PbfConsts c,
       const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
       const float d2 = dist2(dx, dy, dz);
       if (d2 <= c.rmax2 && T[b] != LF_OBSTACLE) {
-        const float w = poly6W(c, __builtin_sqrtf(d2));
+        const float mw = pb.w * poly6W(c, __builtin_sqrtf(d2));
         const float4 vb = Vin[b];
-        ax = __builtin_fmaf(w, vb.x - va.x, ax);
-        ay = __builtin_fmaf(w, vb.y - va.y, ay);
-        az = __builtin_fmaf(w, vb.z - va.z, az);
+        ax = __builtin_fmaf(mw, vb.x - va.x, ax);
+        ay = __builtin_fmaf(mw, vb.y - va.y, ay);
+        az = __builtin_fmaf(mw, vb.z - va.z, az);
       }
     }
   });
-  Vout[a] = make_float4(__builtin_fmaf(c.xsph, ax, va.x), __builtin_fmaf(c.xsph, ay, va.y),
-                        __builtin_fmaf(c.xsph, az, va.z), va.w);
+  Vout[a] = make_float4(__builtin_fmaf(c.xsphRho, ax, va.x), __builtin_fmaf(c.xsphRho, ay, va.y),
+                        __builtin_fmaf(c.xsphRho, az, va.z), va.w);
 }
 
 __device__ inline void unpackARGB(uint32_t c, float out[4]) {

@@ -5,10 +5,14 @@ samples/main.cpp:36-56, x outermost, z innermost, ids in that order) plus the
 solver parameters. The jitter is a counter-based hash of (seed, particle id,
 axis), so any consumer regenerates bit-identical inputs.
 
-Scaling: the recipe at spacing s keeps libfluid's rest-state ratio by giving
-each particle the mass (s / 0.05)^3 (1.0 at the 8,000-particle, s = 0.05
-scene), so every scene starts at the same density relative to the rest
-density 6378 (oclsph_kernel.h:21) and h = 2 s gives ~33 lattice neighbours.
+Scaling: every scene is the same dimensionless recipe in solver units (the
+reference's own Config::scale, world = solver * scale, oclsph.hpp:301,374,414):
+solver spacing 0.05, h = 0.1, mass 1, gravity -9.8, dt 1/120, the regime the
+reference's constants (rest density 6378, CFM epsilon 600, a mass-free
+lambda gradient, oclsph_kernel.h:20-31,323-331) are tuned for. A world
+spacing s is reached with scale = s / 0.05, so h_world = 2 s (~33 lattice
+neighbours). Shrinking h in solver units instead would leave epsilon and the
+mass-free gradient at the wrong scale and the step diverges.
 """
 import dataclasses
 from typing import Sequence, Tuple
@@ -16,7 +20,8 @@ from typing import Sequence, Tuple
 import numpy as np
 
 REST_DENSITY = 6378.0
-BASE_SPACING = 0.05
+SOLVER_SPACING = 0.05
+SOLVER_H = 0.1
 
 _M1 = np.uint64(0x9E3779B97F4A7C15)
 _M2 = np.uint64(0xBF58476D1CE4E5B9)
@@ -50,8 +55,9 @@ def lattice_block(dims, spacing, origin, jitter, seed, id_offset=0):
 @dataclasses.dataclass
 class Scene:
     name: str
-    h: float
-    spacing: float
+    h: float            # solver units
+    spacing: float      # world units
+    scale: float        # world = solver * scale
     position: np.ndarray
     velocity: np.ndarray
     mass: np.ndarray
@@ -75,7 +81,7 @@ class Scene:
 
     def params(self, flags=0, xsph_c=None, iterations=None):
         from . import pbf_params
-        return pbf_params(self.h, dt=self.dt, scale=1.0,
+        return pbf_params(self.h, dt=self.dt, scale=self.scale,
                           iterations=self.iterations if iterations is None else iterations,
                           gravity=self.gravity, min_bound=self.min_bound,
                           max_bound=self.max_bound,
@@ -97,9 +103,9 @@ def _scene(name, blocks, spacing, box, seed, steps, mc_samples, colour=0xFF2D101
     pos = np.concatenate(pos)
     n = pos.shape[0]
     return Scene(
-        name=name, h=2.0 * spacing, spacing=spacing, position=pos,
+        name=name, h=SOLVER_H, spacing=spacing, scale=spacing / SOLVER_SPACING, position=pos,
         velocity=np.zeros((n, 3), np.float32),
-        mass=np.full(n, (spacing / BASE_SPACING) ** 3, np.float32),
+        mass=np.ones(n, np.float32),
         type=np.zeros(n, np.int32), colour=np.full(n, colour, np.uint32),
         ids=np.concatenate(ids), min_bound=(0.0, 0.0, 0.0), max_bound=tuple(box),
         steps=steps, mc_samples=mc_samples, seed=seed)
@@ -114,16 +120,18 @@ def dam_break_8k(seed=1):
 
 
 def dam_break_1m(seed=2):
-    """1,048,576 particles: 128^3 lattice, spacing 0.0125, 4x2x2 box, h = 0.025,
-    100 steps, marching cubes at 256^3 every step."""
+    """1,048,576 particles: a 128 (x) x 128 (y) x 64 (z) lattice column, world
+    spacing 0.0125 (scale 0.25), 4x2x2 box, h = 0.025 world, 100 steps, marching
+    cubes at 256^3 every step. (A 128^3 lattice would hold 2,097,152 particles;
+    the 1,048,576 count is kept.)"""
     s = 0.0125
-    return _scene("dam_break_1m", [((128, 128, 128), (s / 2,) * 3)], s, (4.0, 2.0, 2.0), seed,
+    return _scene("dam_break_1m", [((128, 128, 64), (s / 2,) * 3)], s, (4.0, 2.0, 2.0), seed,
                   100, (256, 256, 256))
 
 
 def double_dam_break_4m(seed=4):
-    """4,194,304 particles: two 128^3 blocks, spacing 0.00625, h = 0.0125, at
-    opposite x ends of a 4x2x2 box, 20 steps."""
+    """4,194,304 particles: two 128^3 blocks, world spacing 0.00625 (scale
+    0.125), h = 0.0125 world, at opposite x ends of a 4x2x2 box, 20 steps."""
     s = 0.00625
     side = 128 * s
     return _scene("double_dam_break_4m",
@@ -138,9 +146,9 @@ def small_block(n_side=10, spacing=0.05, seed=7, box=(1.0, 1.0, 1.0), jitter=Tru
     p, ids = lattice_block((n_side,) * 3, s, (s / 2 + 0.1,) * 3, 0.1 * s if jitter else 0.0,
                            seed)
     n = p.shape[0]
-    return Scene(name="small_block_%d" % n, h=2 * s, spacing=s, position=p,
-                 velocity=np.zeros((n, 3), np.float32),
-                 mass=np.full(n, (s / BASE_SPACING) ** 3, np.float32),
+    return Scene(name="small_block_%d" % n, h=SOLVER_H, spacing=s, scale=s / SOLVER_SPACING,
+                 position=p, velocity=np.zeros((n, 3), np.float32),
+                 mass=np.ones(n, np.float32),
                  type=np.zeros(n, np.int32), colour=np.full(n, 0xFF2D1010, np.uint32), ids=ids,
                  min_bound=(0.0, 0.0, 0.0), max_bound=tuple(box), steps=2,
                  mc_samples=(32, 32, 32), seed=seed)

@@ -94,7 +94,7 @@ template <typename F> inline uint32_t cellCoord(F q) {
 
 template <typename F> struct Consts {
   F h, hh, poly6, spiky, rhoRecip, wdq, invDt, minExt[3], maxExt[3], minB[3],
-      maxB[3], scale, dt, rho0, cfm, corrK, vd, xsph;
+      maxB[3], scale, dt, rho0, cfm, corrK, vd, xsph, xsphRho;
   uint32_t extent[3], G, iterations, flags;
   explicit Consts(const lf_pbf_params& p) {
     h = F(p.h);
@@ -114,6 +114,7 @@ template <typename F> struct Consts {
     corrK = F(p.corr_k);
     vd = F(p.vd);
     xsph = F(p.xsph_c);
+    xsphRho = xsph / rho0;
     iterations = p.iterations;
     flags = p.flags;
     const F pad = h * F(2); /* oclsph.hpp:340 */
@@ -387,8 +388,10 @@ template <typename F> class Pbf {
     }
   }
 
-  /* XSPH viscosity (absent in the reference; Macklin & Mueller 2013 eq. 17):
-   * v_a += c * sum_b (v_b - v_a) W6(|p_a - p_b|) over fluid b, Jacobi. */
+  /* XSPH viscosity (absent in the reference; Macklin & Mueller 2013 eq. 17
+   * with the SPH volume m_b / rho0 on each term, so c is a dimensionless
+   * smoothing fraction): v_a += (c / rho0) sum_b m_b (v_b - v_a) W6(|p_a - p_b|)
+   * over fluid b, Jacobi. */
   void xsphPass() {
     if (c.xsph == F(0)) return;
     std::vector<F> out(vs);
@@ -405,10 +408,10 @@ template <typename F> class Pbf {
         const F dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
         const F d = std::sqrt(dist2(dx, dy, dz));
         if (!(d <= c.h)) return;
-        const F w = c.W6(d);
-        for (int k = 0; k < 3; ++k) acc[k] = std::fma(w, vs[3 * b + k] - va[k], acc[k]);
+        const F mw = ms[b] * c.W6(d);
+        for (int k = 0; k < 3; ++k) acc[k] = std::fma(mw, vs[3 * b + k] - va[k], acc[k]);
       });
-      for (int k = 0; k < 3; ++k) out[3 * a + k] = std::fma(c.xsph, acc[k], va[k]);
+      for (int k = 0; k < 3; ++k) out[3 * a + k] = std::fma(c.xsphRho, acc[k], va[k]);
     }
     vs.swap(out);
   }

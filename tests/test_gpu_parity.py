@@ -300,3 +300,30 @@ def test_sort_properties_random_keys(ctx):
     G = table.shape[0]
     assert np.array_equal(table, np.searchsorted(keys, np.arange(G, dtype=np.uint32),
                                                  side="left").astype(np.uint32))
+
+
+def test_1m_hundred_steps_stay_physical(ctx):
+    """100 steps of the 1,048,576-particle dam break: finite state inside the
+    box, bounded cell occupancy, densities near the rest density, a closed
+    surface mesh every step."""
+    sc = scenes.dam_break_1m()
+    p = sc.params()
+    m = sc.mc()
+    ctx.upload(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
+    for _ in range(10):
+        ctx.step(p, 10)
+        ctx.mc_sample(p, m)
+        ctx.mc_classify_count(m)
+        ctx.mc_emit(p, m)
+        ntri = ctx.mc_count()
+        assert 10_000 < ntri < 16_000_000
+    g = ctx.download()
+    assert np.isfinite(g["position"]).all() and np.isfinite(g["velocity"]).all()
+    assert (g["position"] >= 0).all() and (g["position"] <= np.array([4, 2, 2], np.float32)).all()
+    assert np.abs(g["velocity"]).max() < 50.0
+    table = ctx.read(LF_ARR_TABLE)
+    assert np.diff(table.astype(np.int64)).max() < 64
+    ctx.lambda_pass(p)
+    rho = ctx.read(LF_ARR_DENSITY)
+    assert 5000 < np.median(rho) < 8500
+    assert sorted(g["order"].tolist()) == list(range(sc.n))
