@@ -207,6 +207,9 @@ int lf_advance(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m,
 #define LF_ARR_MCCOLOUR 12  /* u32[S]   MC field colour (ARGB bytes) */
 #define LF_ARR_CASES 13     /* u8[M]    MC cube case index */
 #define LF_ARR_DENSITY 14   /* f32[n]   density of the last lambda pass */
+#define LF_ARR_LISTSTATS 15 /* u32[4]   neighbour lists: [0] float bits of the largest
+                               displacement since the build, [1..3] particles that
+                               enumerated cells instead (lambda, delta, XSPH) */
 int lf_read(lf_ctx* ctx, int array, void* dst, size_t bytes);
 /* Keep the case index of every cube for lf_read(LF_ARR_CASES) (off by
  * default: 1 byte per cube of extra traffic). */

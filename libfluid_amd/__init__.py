@@ -250,6 +250,8 @@ class Context:
             out = np.empty(n, np.uint8)
         elif array == LF_ARR_DENSITY:
             out = np.empty(n, np.float32)
+        elif array == _abi.LF_ARR_LISTSTATS:
+            out = np.empty(4, np.uint32)
         elif array == LF_ARR_FIELD:
             S = self.mc_info()[0]
             out = np.empty(S, np.float32)

@@ -41,6 +41,7 @@ LF_ARR_MCNORMAL = 11
 LF_ARR_MCCOLOUR = 12
 LF_ARR_CASES = 13
 LF_ARR_DENSITY = 14
+LF_ARR_LISTSTATS = 15
 
 _f3 = ctypes.c_float * 3
 _u3 = ctypes.c_uint32 * 3

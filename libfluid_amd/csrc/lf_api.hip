@@ -30,7 +30,7 @@ static float hostW6(const PbfConsts& c, float r) {
 
 // Folds the reference's kernel constants (oclsph_kernel.h:20-51) and the grid
 // geometry of computeBoundAndZindex (oclsph.hpp:334-387) in FP32.
-int foldConsts(const lf_pbf_params& p, PbfConsts& c, std::string& err) {
+int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string& err) {
   if (!(p.h > 0.0f) || !(p.dt > 0.0f) || !(p.scale > 0.0f) || !(p.rest_density > 0.0f)) {
     err = "lf_pbf_params: h, dt, scale and rest_density must be > 0";
     return LF_ERR_ARG;
@@ -51,6 +51,12 @@ int foldConsts(const lf_pbf_params& p, PbfConsts& c, std::string& err) {
   c.vd = p.vd;
   c.xsph = p.xsph_c;
   c.xsphRho = c.xsph / c.rho0;
+  {
+    const double skin = double(skinFrac) * double(c.h);
+    const double r = double(c.h) + skin;
+    c.listR2 = float(r * r * (1.0 + 1e-5));
+    c.skinSafe = float(skin * (1.0 - 1e-5));
+  }
   c.flags = p.flags;
   c.wdq = hostW6(c, p.corr_dq * c.h);
   // rmax2: the largest float x with sqrtf(x) <= h
@@ -116,7 +122,7 @@ static int setParams(Ctx* c, const lf_pbf_params* p) {
   if (!p) return fail(c, LF_ERR_ARG, "null lf_pbf_params");
   PbfConsts k{};
   std::string err;
-  const int rc = foldConsts(*p, k, err);
+  const int rc = foldConsts(*p, c->skinFrac, k, err);
   if (rc) return fail(c, rc, err);
   c->consts = k;
   return LF_OK;
@@ -132,6 +138,7 @@ static int requireGrid(Ctx* c, const char* who) {
 static int pbfStepOnce(Ctx* c) {
   int rc;
   if ((rc = gridBuild(c))) return rc;
+  if ((rc = pbfBuildLists(c))) return rc;
   if (c->consts.flags & LF_FLAG_DIFFUSE)
     if ((rc = pbfDiffuse(c))) return rc;
   for (uint32_t it = 0; it < c->params.iterations; ++it) {
@@ -207,7 +214,11 @@ int lf_ctx_destroy(lf_ctx* c) {
   }
   lf::release(c->P);
   lf::release(c->PL);
-  lf::release(c->P2);
+  lf::release(c->nbr);
+  lf::release(c->nbrCount);
+  lf::release(c->P0);
+  lf::release(c->maxDisp);
+  lf::release(c->cellDisp);
   lf::release(c->rho);
   lf::release(c->diffused);
   lf::release(c->table);
@@ -337,7 +348,8 @@ int lf_grid_build(lf_ctx* c, const lf_pbf_params* p) {
   c->params = *p;
   LF_HIP(c, hipSetDevice(c->device));
   c->stepDone = false;
-  return lf::gridBuild(c);
+  if ((rc = lf::gridBuild(c))) return rc;
+  return lf::pbfBuildLists(c);
 }
 
 #define LF_STAGE(fn, who)                                  \
@@ -526,6 +538,7 @@ int lf_read(lf_ctx* c, int array, void* dst, size_t bytes) {
     case LF_ARR_TYPE: src = c->T[c->cur].p; avail = n; break;
     case LF_ARR_DIFFUSED: src = c->diffused.p; avail = n * 4; break;
     case LF_ARR_DENSITY: src = c->rho.p; avail = n * 4; break;
+    case LF_ARR_LISTSTATS: src = c->maxDisp.p; avail = c->maxDisp.p ? 16 : 0; break;
     case LF_ARR_FIELD: src = c->field.p; avail = size_t(c->S[0]) * c->S[1] * c->S[2] * 4; break;
     case LF_ARR_MCNORMAL: src = c->mcNormal.p; avail = size_t(c->S[0]) * c->S[1] * c->S[2] * 16; break;
     case LF_ARR_MCCOLOUR: src = c->mcColour.p; avail = size_t(c->S[0]) * c->S[1] * c->S[2] * 4; break;
@@ -561,6 +574,12 @@ int lf_write(lf_ctx* c, int array, const void* src, size_t bytes) {
   if (bytes > avail || (bytes && !dst)) return fail(c, LF_ERR_ARG, "lf_write: more bytes than the array holds");
   if (bytes) {
     LF_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    if ((array == LF_ARR_PSTAR || array == LF_ARR_LAMBDA) && c->maxDisp.p) {
+      // positions changed outside the delta pass: no neighbour list is usable
+      static const uint32_t kInf = 0x7F800000u;
+      LF_HIP(c, hipMemcpyAsync(c->maxDisp.p, &kInf, 4, hipMemcpyHostToDevice, c->stream));
+      c->listFresh = false;
+    }
     LF_HIP(c, hipStreamSynchronize(c->stream));
   }
   return LF_OK;

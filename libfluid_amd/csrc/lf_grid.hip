@@ -73,7 +73,8 @@ constexpr uint32_t kPerThread = kCells / kThreads;
 // LDS, then a suffix-min fills empty cells with the next start.
 __global__ __launch_bounds__(kThreads) void k_cell_table(const uint32_t* __restrict__ keys,
                                                          uint32_t n, uint32_t G,
-                                                         uint32_t* __restrict__ table) {
+                                                         uint32_t* __restrict__ table,
+                                                         uint32_t* __restrict__ cellDisp) {
   __shared__ uint32_t local[kCells];
   __shared__ uint32_t range[2];
   __shared__ uint32_t thrMin[kThreads];
@@ -106,7 +107,10 @@ __global__ __launch_bounds__(kThreads) void k_cell_table(const uint32_t* __restr
 #pragma unroll
   for (uint32_t j = 0; j < kPerThread; ++j) {
     const uint32_t cell = c0 + lo + j;
-    if (cell < c1) table[cell] = min(v[j], after);
+    if (cell < c1) {
+      table[cell] = min(v[j], after);
+      cellDisp[cell] = 0u;  // per-cell displacement of the neighbour lists
+    }
   }
 }
 
@@ -166,10 +170,14 @@ int gridBuild(Ctx* c) {
   if ((rc = ensure(c, c->ID[nxt], n, "ids"))) return rc;
   if ((rc = ensure(c, c->P, n, "p*"))) return rc;
   if ((rc = ensure(c, c->PL, n, "p*+lambda"))) return rc;
-  if ((rc = ensure(c, c->P2, n, "p* spare"))) return rc;
+  if ((rc = ensure(c, c->nbr, nbrCapacity(n), "neighbour lists"))) return rc;
+  if ((rc = ensure(c, c->nbrCount, n, "neighbour counts"))) return rc;
+  if ((rc = ensure(c, c->P0, n, "list positions"))) return rc;
+  if ((rc = ensure(c, c->maxDisp, 4, "displacement"))) return rc;
   if ((rc = ensure(c, c->rho, n, "density"))) return rc;
   if ((rc = ensure(c, c->diffused, n, "diffused"))) return rc;
   if ((rc = ensure(c, c->table, size_t(k.G), "cell table"))) return rc;
+  if ((rc = ensure(c, c->cellDisp, size_t(k.G), "cell displacement"))) return rc;
   if (n == 0) return LF_OK;
   const unsigned nb = blocksFor(n, kThreads);
   rc = launch(c, "predict_key", k_predict_key, dim3(nb), dim3(kThreads), 0, uint32_t(n), k,
@@ -186,7 +194,7 @@ int gridBuild(Ctx* c) {
               c->X[nxt].p, c->V[nxt].p, c->T[nxt].p, c->ID[nxt].p, c->P.p);
   if (rc) return rc;
   rc = launch(c, "cell_table", k_cell_table, dim3(blocksFor(k.G, kCells)), dim3(kThreads), 0,
-              (const uint32_t*)c->sortKey[idx].p, uint32_t(n), k.G, c->table.p);
+              (const uint32_t*)c->sortKey[idx].p, uint32_t(n), k.G, c->table.p, c->cellDisp.p);
   if (rc) return rc;
   c->keysIdx = idx;
   c->gridG = k.G;

@@ -36,6 +36,8 @@ namespace lf {
 struct PbfConsts {
   float h, hh, rmax2, poly6, spiky, rhoRecip, rho0, cfm, corrK, wdq, invDt, dt, scale, vd,
       xsph, xsphRho;  // xsphRho = xsph / rho0
+  float listR2;      // neighbour-list radius^2, (h + skin)^2 with rounding margin
+  float skinSafe;    // skin less a rounding margin
   float minExt[3];   // minBound/scale - 2h (oclsph.hpp:359)
   float minB[3], maxB[3];
   float grav[3];
@@ -44,7 +46,20 @@ struct PbfConsts {
   uint32_t flags;
 };
 
-int foldConsts(const lf_pbf_params& p, PbfConsts& c, std::string& err);
+int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string& err);
+
+// Neighbour-list capacity per particle (~50 within h + 0.2h at rest with
+// h = 2 spacings). A particle with more keeps the count and enumerates its 27
+// cells instead; results are the same either way.
+constexpr uint32_t kMaxNbr = 128;
+
+// Slot of entry i of particle a's list: wave-chunked, entry-major within a
+// chunk of 64 particles, so the lanes of a wave reading entry i touch one
+// contiguous 256-byte run, and one wave's lists (24 KB) stay together.
+__host__ __device__ inline uint64_t nbrSlot(uint32_t a, uint32_t i) {
+  return ((uint64_t(a >> 6) * kMaxNbr + i) << 6) | (a & 63u);
+}
+inline size_t nbrCapacity(size_t n) { return ((n + 63) / 64) * 64 * kMaxNbr; }
 
 // ---- Morton codes (curves.h:56-72), 10 bits per axis ----------------------
 __host__ __device__ inline uint32_t spread10(uint32_t v) {
@@ -187,8 +202,13 @@ struct Ctx {
   DevBuf<uint8_t> T[2];       // particle type
   DevBuf<uint32_t> ID[2];     // upload index
   DevBuf<uint32_t> sortKey[2], sortVal[2];
-  DevBuf<float4> P, PL, P2;   // p* (+mass), p* + lambda, spare
+  DevBuf<float4> P, PL;       // p* (+mass), p* + lambda
   DevBuf<float> rho;          // density of the last lambda pass
+  DevBuf<uint32_t> nbr;       // neighbour lists, nbrSlot(a, i)
+  DevBuf<uint32_t> nbrCount;  // list length per particle (> kMaxNbr: overflowed)
+  DevBuf<float4> P0;          // p* at the list build
+  DevBuf<uint32_t> maxDisp;   // [0]: float bits of the largest displacement since
+  DevBuf<uint32_t> cellDisp;  // per cell: float bits of its particles' largest one
   DevBuf<uint32_t> diffused;
   DevBuf<uint32_t> table;
   DevBuf<uint32_t> sortHist;
@@ -198,6 +218,8 @@ struct Ctx {
 
   bool gridValid = false;
   bool stepDone = false;
+  float skinFrac = 0.2f;   // neighbour-list skin / h
+  bool listFresh = false;  // P still holds the predicted p* the lists were built from
   PbfConsts consts{};
   lf_pbf_params params{};
   uint32_t gridG = 0;  // table size of the last grid build
@@ -277,7 +299,8 @@ inline unsigned blocksFor(size_t n, unsigned per) { return unsigned((n + per - 1
 // ---- launchers (one .hip file each) ----------------------------------------
 int gridBuild(Ctx* c);                        // lf_grid.hip
 int radixSort(Ctx* c, size_t n, int& outIdx);  // lf_sort.hip
-int pbfLambda(Ctx* c);                        // lf_pbf.hip
+int pbfBuildLists(Ctx* c);                    // lf_pbf.hip
+int pbfLambda(Ctx* c);
 int pbfDelta(Ctx* c);
 int pbfFinalise(Ctx* c);
 int pbfXsph(Ctx* c);

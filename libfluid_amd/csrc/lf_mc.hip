@@ -46,29 +46,165 @@ struct Lattice {
 
 // ---- field sampling -----------------------------------------------------------
 
-// SPH density field rho(x) = sum over fluid b of m_b W6(|x - p*_b|); P.w is the
-// fluid mass (0 for obstacles) after finalise.
-__global__ __launch_bounds__(kThreads) void k_mc_density(uint64_t total, Lattice L, PbfConsts c,
+// SPH density field rho(x) = sum over fluid b of m_b poly6 (h^2 - d^2)^3 for
+// d^2 = |x - p*_b|^2 <= h^2 (W6 written on d^2, no square root); P.w is the
+// fluid mass (0 for obstacles) after finalise. Candidates: the 27 cells around
+// the sample's own cell, reference order.
+//
+// One workgroup per kTX x kTY x kTZ tile of samples: the particles of the
+// tile's halo cells (its cell range +-1 per axis) are staged once in LDS with
+// a local cell table, so the per-sample candidate loop reads LDS; a tile whose
+// halo holds no particle writes zeros and exits. Tiles whose halo does not fit
+// (coarse lattices, crowded cells) read global memory instead, same order.
+constexpr int kTX = 4, kTY = 8, kTZ = 8;  // 256 samples
+constexpr int kHaloCells = 512;
+constexpr int kStageCap = 2048;           // staged particles (32 KB)
+
+__device__ inline float densityTerm(const PbfConsts& c, float4 pb, float px, float py, float pz) {
+  const float d2 = dist2(px - pb.x, py - pb.y, pz - pb.z);
+  if (!(d2 <= c.hh)) return 0.0f;
+  const float t = c.hh - d2;
+  return pb.w * (c.poly6 * ((t * t) * t));
+}
+
+__device__ inline void cellCoords(const PbfConsts& c, float px, float py, float pz, uint32_t& x,
+                                  uint32_t& y, uint32_t& z) {
+  x = cellCoord((px - c.minExt[0]) / c.h);
+  y = cellCoord((py - c.minExt[1]) / c.h);
+  z = cellCoord((pz - c.minExt[2]) / c.h);
+}
+
+__global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
                                                          const uint32_t* __restrict__ table,
                                                          const float4* __restrict__ P,
-                                                         float* __restrict__ field) {
-  const uint64_t idx = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (idx >= total) return;
-  const uint32_t z = uint32_t(idx % L.S[2]);
-  const uint64_t xy = idx / L.S[2];
-  const uint32_t y = uint32_t(xy % L.S[1]);
-  const uint32_t x = uint32_t(xy / L.S[1]);
+                                                         float* __restrict__ field,
+                                                         uint32_t tilesY, uint32_t tilesZ) {
+  __shared__ float4 stage[kStageCap];
+  __shared__ uint32_t cellStart[kHaloCells + 1];
+  __shared__ uint32_t scanTmp[kThreads / 64];
+  __shared__ int mode;  // 0 LDS, 1 global
+  const uint32_t tz = blockIdx.x % tilesZ;
+  const uint32_t ty = (blockIdx.x / tilesZ) % tilesY;
+  const uint32_t tx = blockIdx.x / (tilesZ * tilesY);
+  const uint32_t x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+  const uint32_t x1 = min(x0 + kTX, L.S[0]) - 1, y1 = min(y0 + kTY, L.S[1]) - 1,
+                 z1 = min(z0 + kTZ, L.S[2]) - 1;
+  // cell range of the tile (cellCoord is monotone in the sample position)
+  uint32_t lo[3], hi[3];
+  cellCoords(c, __builtin_fmaf(float(x0), L.step[0], L.origin[0]),
+             __builtin_fmaf(float(y0), L.step[1], L.origin[1]),
+             __builtin_fmaf(float(z0), L.step[2], L.origin[2]), lo[0], lo[1], lo[2]);
+  cellCoords(c, __builtin_fmaf(float(x1), L.step[0], L.origin[0]),
+             __builtin_fmaf(float(y1), L.step[1], L.origin[1]),
+             __builtin_fmaf(float(z1), L.step[2], L.origin[2]), hi[0], hi[1], hi[2]);
+  bool fits = true;
+  uint32_t hx = 0, hy = 0, hz = 0;
+  for (int k = 0; k < 3; ++k) fits = fits && lo[k] >= 1 && hi[k] <= 1022;
+  if (fits) {
+    hx = hi[0] - lo[0] + 3;
+    hy = hi[1] - lo[1] + 3;
+    hz = hi[2] - lo[2] + 3;
+    fits = hx * hy * hz <= uint32_t(kHaloCells);
+  }
+  const uint32_t nh = fits ? hx * hy * hz : 0;
+  // count the halo cells' particles (per thread: cells tid, tid+256)
+  uint32_t myCount[2] = {0, 0};
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t h = threadIdx.x + r * kThreads;
+    if (h < nh) {
+      const uint32_t ix = h % hx, iy = (h / hx) % hy, iz = h / (hx * hy);
+      const uint32_t code = mortonEncode(lo[0] - 1 + ix, lo[1] - 1 + iy, lo[2] - 1 + iz);
+      if (code + 1u < c.G) myCount[r] = table[code + 1u] - table[code];
+    }
+  }
+  // exclusive scan over cells in order h = 0..nh-1 (cells tid and tid+256)
+  uint32_t total0, total1;
+  {
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t v = myCount[0], x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) scanTmp[w] = x;
+    __syncthreads();
+    uint32_t wb = 0;
+    total0 = 0;
+    for (int i = 0; i < kThreads / 64; ++i) {
+      if (i < w) wb += scanTmp[i];
+      total0 += scanTmp[i];
+    }
+    const uint32_t ex0 = wb + x - v;
+    __syncthreads();
+    v = myCount[1];
+    x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) scanTmp[w] = x;
+    __syncthreads();
+    wb = 0;
+    total1 = 0;
+    for (int i = 0; i < kThreads / 64; ++i) {
+      if (i < w) wb += scanTmp[i];
+      total1 += scanTmp[i];
+    }
+    const uint32_t ex1 = total0 + wb + x - v;
+    if (threadIdx.x < nh) cellStart[threadIdx.x] = ex0;
+    if (threadIdx.x + kThreads < nh) cellStart[threadIdx.x + kThreads] = ex1;
+    if (threadIdx.x == 0) {
+      cellStart[nh] = total0 + total1;
+      mode = (!fits || total0 + total1 > uint32_t(kStageCap)) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  const uint32_t totalParticles = total0 + total1;
+  const bool useLds = mode == 0;
+  if (useLds) {
+    for (int r = 0; r < 2; ++r) {
+      const uint32_t h = threadIdx.x + r * kThreads;
+      if (h < nh && myCount[r] > 0) {
+        const uint32_t ix = h % hx, iy = (h / hx) % hy, iz = h / (hx * hy);
+        const uint32_t code = mortonEncode(lo[0] - 1 + ix, lo[1] - 1 + iy, lo[2] - 1 + iz);
+        const uint32_t s0 = table[code], dst = cellStart[h];
+        for (uint32_t j = 0; j < myCount[r]; ++j) stage[dst + j] = P[s0 + j];
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t lx = threadIdx.x / (kTY * kTZ), ly = (threadIdx.x / kTZ) % kTY,
+                 lz = threadIdx.x % kTZ;
+  const uint32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
+  if (x >= L.S[0] || y >= L.S[1] || z >= L.S[2]) return;
+  const uint64_t idx = (uint64_t(x) * L.S[1] + y) * L.S[2] + z;
+  if (useLds && totalParticles == 0) {
+    field[idx] = 0.0f;
+    return;
+  }
   const float px = __builtin_fmaf(float(x), L.step[0], L.origin[0]);
   const float py = __builtin_fmaf(float(y), L.step[1], L.origin[1]);
   const float pz = __builtin_fmaf(float(z), L.step[2], L.origin[2]);
   float rho = 0.0f;
-  forEachNeighbourCell(cellKey(c, px, py, pz), table, c.G, [&](uint32_t s, uint32_t e) {
-    for (uint32_t b = s; b < e; ++b) {
-      const float4 pb = P[b];
-      const float d2 = dist2(px - pb.x, py - pb.y, pz - pb.z);
-      if (d2 <= c.rmax2) rho = __builtin_fmaf(pb.w, poly6W(c, __builtin_sqrtf(d2)), rho);
-    }
-  });
+  if (useLds) {
+    uint32_t cx, cy, cz;
+    cellCoords(c, px, py, pz, cx, cy, cz);
+    const uint32_t bx = cx - lo[0], by = cy - lo[1], bz = cz - lo[2];  // halo index of dx=-1
+    for (uint32_t dz = 0; dz < 3; ++dz)
+      for (uint32_t dy = 0; dy < 3; ++dy) {
+        const uint32_t row = ((bz + dz) * hy + (by + dy)) * hx + bx;
+        for (uint32_t dx = 0; dx < 3; ++dx) {
+          const uint32_t e = cellStart[row + dx + 1];
+          for (uint32_t b = cellStart[row + dx]; b < e; ++b)
+            rho += densityTerm(c, stage[b], px, py, pz);
+        }
+      }
+  } else {
+    forEachNeighbourCell(cellKey(c, px, py, pz), table, c.G, [&](uint32_t s, uint32_t e) {
+      for (uint32_t b = s; b < e; ++b)
+        rho += densityTerm(c, P[b], px, py, pz);
+    });
+  }
   field[idx] = rho;
 }
 
@@ -418,8 +554,12 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
                 (const float4*)c->V[c->cur].p, c->field.p, c->mcNormal.p, c->mcColour.p);
     c->mcAttr = 1;
   } else {
-    rc = launch(c, "mc_density_field", k_mc_density, dim3(nb), dim3(kThreads), 0, total, L, k,
-                (const uint32_t*)c->table.p, (const float4*)c->P.p, c->field.p);
+    const uint32_t tX = (L.S[0] + kTX - 1) / kTX, tY = (L.S[1] + kTY - 1) / kTY,
+                   tZ = (L.S[2] + kTZ - 1) / kTZ;
+    const uint64_t tiles = uint64_t(tX) * tY * tZ;
+    if (tiles >= (uint64_t(1) << 31)) return fail(c, LF_ERR_ARG, "lf_mc_sample: lattice too large");
+    rc = launch(c, "mc_density_field", k_mc_density, dim3(unsigned(tiles)), dim3(kThreads), 0, L,
+                k, (const uint32_t*)c->table.p, (const float4*)c->P.p, c->field.p, tY, tZ);
     c->mcAttr = 0;
   }
   if (rc) return rc;

@@ -1,22 +1,223 @@
-// PBF constraint passes on gfx950: density/lambda, position correction
-// (Jacobi), finalise, XSPH viscosity and colour diffusion.
+// PBF constraint passes on gfx950: neighbour lists, density/lambda, position
+// correction (Jacobi), finalise, XSPH viscosity and colour diffusion.
 //
-// One work-item per Morton-sorted particle; candidates come from the 27 cells
-// around the particle's own cell in the reference's order
-// (oclsph_kernel.h:150-192), so each sum runs in the same order as the
-// reference and the CPU restatement.
+// One work-item per Morton-sorted particle. The candidates of a particle are
+// the particles of the 27 cells around its own cell, in the reference's order
+// (oclsph_kernel.h:150-192); only those within h contribute.
+//
+// Neighbour lists: once per step, k_build_list records, in that visit order,
+// the candidates within h + skin of each particle, measured between the
+// predicted p* clamped to the bounds (P0). The lambda, delta and XSPH passes
+// then walk the list with the exact r <= h test instead of the 27 cells.
+//  * First lambda/delta pair (unclamped predicted p*): clamping to a box is
+//    a projection onto a convex set, so |clamp(a) - clamp(b)| <= |a - b| and
+//    every pair within h is listed; the lists are complete.
+//  * Later passes: a candidate left out was farther than h + skin apart in
+//    P0, so it can only come within h if the two particles together moved by
+//    more than skin from P0. The delta pass keeps D, the largest |p* - P0| of
+//    any particle; particle a walks its list only while |p*_a - P0_a| + D <
+//    skin, and otherwise (or when its list overflowed) enumerates its cells.
+// Either way the accepted set and its order are the reference's, so the sums
+// are bit-identical to the cell walk.
 //
 // Buffers (sorted order): P = (p*, mass), PL = (p*, lambda), X = (world
-// position, mass), V = (velocity, colour bits), T = type.
+// position, mass), V = (velocity, colour bits), T = type, P0 = clamped p* at
+// the list build.
 #include "lf_internal.hpp"
 
 namespace lf {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr uint32_t kQueueCap = 24;  // LDS queue entries per lane
+constexpr uint32_t kDrainAt = 8;    // the wave drains once a lane holds this many
 
-// sph_lambda (oclsph_kernel.h:305-332). KAHAN: Neumaier-compensated density
-// and gradient sums (mixed-precision option).
+// Visits the candidates of a particle in the reference order (27 cells of
+// `key`) and passes those with accept(b) to use(b), in visit order, through a
+// per-lane LDS queue (q[i * kThreads]) that the whole wave drains together once
+// any lane holds kDrainAt entries: the stores of use() then issue as a few
+// wide instructions instead of one per candidate under divergence.
+template <typename Accept, typename Use>
+__device__ inline void visitCompacted(uint32_t key, const uint32_t* __restrict__ table, uint32_t G,
+                                      uint32_t* q, Accept&& accept, Use&& use) {
+  uint32_t qn = 0;
+  auto drain = [&]() {
+    for (uint32_t i = 0; i < qn; ++i) use(q[i * kThreads]);
+    qn = 0;
+  };
+  forEachNeighbourCell(key, table, G, [&](uint32_t s, uint32_t e) {
+    for (uint32_t b = s; b < e; ++b) {
+      if (accept(b)) {
+        if (qn == kQueueCap) drain();
+        q[qn * kThreads] = b;
+        ++qn;
+      }
+    }
+    if (__any(qn >= kDrainAt)) drain();
+  });
+  drain();
+}
+
+struct Lists {
+  uint32_t* nbr;           // nbrSlot(a, i)
+  uint32_t* count;         // entries per particle (> kMaxNbr: overflowed)
+  float4* P0;              // clamped p* at the build
+  uint32_t* maxDisp;       // [0]: float bits of D, the largest |p* - P0| so far;
+                           // [1..3]: fallbacks of the lambda, delta, XSPH passes
+  uint32_t* cellDisp;      // per cell (G entries): float bits of the largest |p* - P0|
+                           // of the particles of that cell
+};
+
+// p* clamped to the bounds as the delta pass clamps it (oclsph_kernel.h:364-374).
+__device__ inline float clampAxis(const PbfConsts& c, int k, float p) {
+  float pos = p * c.scale;
+  pos = (c.minB[k] < pos) ? pos : c.minB[k];
+  pos = (pos < c.maxB[k]) ? pos : c.maxB[k];
+  return pos / c.scale;
+}
+
+// Whether particle a may walk its list at position (x, y, z); `fresh` marks
+// the first lambda/delta pair after the build, where the lists are complete.
+__device__ inline bool listUsable(const PbfConsts& c, const Lists& L, uint32_t a, float x, float y,
+                                  float z, bool fresh, uint32_t& cnt) {
+  cnt = L.count[a];
+  if (cnt > kMaxNbr) return false;
+  if (fresh) return true;
+  const float4 p0 = L.P0[a];
+  const float da = __builtin_sqrtf(dist2(x - p0.x, y - p0.y, z - p0.z));
+  return da + __uint_as_float(L.maxDisp[0]) < c.skinSafe;
+}
+
+// The local form of the test: D replaced by the largest displacement among the
+// particles of the 27 cells around `key` (the only possible candidates).
+__device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint32_t a, float x,
+                                       float y, float z, uint32_t key) {
+  const float4 p0 = L.P0[a];
+  const float da = __builtin_sqrtf(dist2(x - p0.x, y - p0.y, z - p0.z));
+  uint32_t m = 0;  // float bits of non-negative floats order as integers
+  const uint32_t xs[3] = {dilatedDec(key, kMortonX), key & kMortonX, dilatedInc(key, kMortonX)};
+  const uint32_t ys[3] = {dilatedDec(key, kMortonY), key & kMortonY, dilatedInc(key, kMortonY)};
+  const uint32_t zs[3] = {dilatedDec(key, kMortonZ), key & kMortonZ, dilatedInc(key, kMortonZ)};
+#pragma unroll
+  for (int iz = 0; iz < 3; ++iz)
+#pragma unroll
+    for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+      for (int ix = 0; ix < 3; ++ix) {
+        const uint32_t code = xs[ix] | ys[iy] | zs[iz];
+        if (code + 1u < c.G) m = max(m, L.cellDisp[code]);
+      }
+  return da + __uint_as_float(m) < c.skinSafe;
+}
+
+// Neighbour lists of the predicted p* (see the file comment).
+__global__ __launch_bounds__(kThreads) void k_build_list(uint32_t n, PbfConsts c,
+                                                         const uint32_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ table,
+                                                         const uint8_t* __restrict__ T,
+                                                         const float4* __restrict__ P, Lists L) {
+  __shared__ uint32_t queue[kQueueCap * kThreads];
+  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
+  if (a < 4) L.maxDisp[a] = 0u;
+  if (a >= n) return;
+  const float4 q = P[a];
+  const float4 pa = make_float4(clampAxis(c, 0, q.x), clampAxis(c, 1, q.y), clampAxis(c, 2, q.z), q.w);
+  L.P0[a] = pa;
+  if (T[a] == LF_OBSTACLE) {
+    L.count[a] = 0;
+    return;
+  }
+  uint32_t cnt = 0;
+  visitCompacted(
+      keys[a], table, c.G, queue + threadIdx.x,
+      [&](uint32_t b) {
+        const float4 pb = P[b];
+        return dist2(pa.x - clampAxis(c, 0, pb.x), pa.y - clampAxis(c, 1, pb.y),
+                     pa.z - clampAxis(c, 2, pb.z)) <= c.listR2;
+      },
+      [&](uint32_t b) {
+        if (cnt < kMaxNbr) L.nbr[nbrSlot(a, cnt)] = b;
+        ++cnt;
+      });
+  L.count[a] = cnt;
+}
+
+// Accumulators of the lambda pass; KAHAN adds Neumaier compensation terms.
+struct LambdaSums {
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f, rho = 0.0f;
+  float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, crho = 0.0f;
+};
+
+__device__ inline void neumaier(float& sum, float& comp, float term) {
+  const float t = sum + term;
+  comp += (fabsf(sum) >= fabsf(term)) ? (sum - t) + term : (term - t) + sum;
+  sum = t;
+}
+
+// One accepted pair of sph_lambda (oclsph_kernel.h:324-326): gradient sum
+// g += gradW / rho0 and density rho += m_a W6(r).
+template <bool KAHAN>
+__device__ inline void lambdaTerm(const PbfConsts& c, float ma, float dx, float dy, float dz,
+                                  float d2, LambdaSums& A) {
+  const float r = __builtin_sqrtf(d2);
+  const float w = poly6W(c, r);
+  const float sc = spikyScale(c, r);
+  if (!KAHAN) {
+    A.gx = __builtin_fmaf(dx * sc, c.rhoRecip, A.gx);
+    A.gy = __builtin_fmaf(dy * sc, c.rhoRecip, A.gy);
+    A.gz = __builtin_fmaf(dz * sc, c.rhoRecip, A.gz);
+    A.rho = __builtin_fmaf(ma, w, A.rho);
+  } else {
+    neumaier(A.gx, A.cgx, (dx * sc) * c.rhoRecip);
+    neumaier(A.gy, A.cgy, (dy * sc) * c.rhoRecip);
+    neumaier(A.gz, A.cgz, (dz * sc) * c.rhoRecip);
+    neumaier(A.rho, A.crho, ma * w);
+  }
+}
+
+// Visits the accepted neighbours (r <= h) of particle a at position pa, in
+// the reference's order: through its list when usable, else the 27 cells.
+// load(b) returns the candidate's float4 (xyz = p*).
+template <typename Load, typename Use>
+__device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int pass, bool fresh,
+                                       uint32_t a, float4 pa, uint32_t key,
+                                       const uint32_t* __restrict__ table, Load&& load,
+                                       Use&& use) {
+  uint32_t cnt;
+  bool usable = listUsable(c, L, a, pa.x, pa.y, pa.z, fresh, cnt);
+  if (!usable && cnt <= kMaxNbr) usable = listUsableLocal(c, L, a, pa.x, pa.y, pa.z, key);
+  const uint64_t fell = __ballot(!usable);
+  if (fell && __lane_id() == __builtin_ctzll(fell)) atomicAdd(&L.maxDisp[pass], uint32_t(__popcll(fell)));
+  if (usable) {
+    uint32_t i = 0;
+    for (; i + 2 <= cnt; i += 2) {  // two gathers in flight
+      const uint32_t b0 = L.nbr[nbrSlot(a, i)], b1 = L.nbr[nbrSlot(a, i + 1)];
+      const float4 p0 = load(b0), p1 = load(b1);
+      const float d0 = dist2(pa.x - p0.x, pa.y - p0.y, pa.z - p0.z);
+      const float d1 = dist2(pa.x - p1.x, pa.y - p1.y, pa.z - p1.z);
+      if (d0 <= c.rmax2) use(b0, p0, d0);
+      if (d1 <= c.rmax2) use(b1, p1, d1);
+    }
+    if (i < cnt) {
+      const uint32_t b0 = L.nbr[nbrSlot(a, i)];
+      const float4 p0 = load(b0);
+      const float d0 = dist2(pa.x - p0.x, pa.y - p0.y, pa.z - p0.z);
+      if (d0 <= c.rmax2) use(b0, p0, d0);
+    }
+  } else {
+    forEachNeighbourCell(key, table, c.G, [&](uint32_t s, uint32_t e) {
+      for (uint32_t b = s; b < e; ++b) {
+        const float4 pb = load(b);
+        const float d2 = dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z);
+        if (d2 <= c.rmax2) use(b, pb, d2);
+      }
+    });
+  }
+}
+
+// sph_lambda (oclsph_kernel.h:305-332). Rejected candidates contribute
+// exactly zero in the reference, so walking only the accepted ones gives the
+// same sums in the same order.
 template <bool KAHAN>
 __global__ __launch_bounds__(kThreads) void k_lambda(uint32_t n, PbfConsts c,
                                                      const uint32_t* __restrict__ keys,
@@ -24,7 +225,8 @@ __global__ __launch_bounds__(kThreads) void k_lambda(uint32_t n, PbfConsts c,
                                                      const uint8_t* __restrict__ T,
                                                      const float4* __restrict__ P,
                                                      float4* __restrict__ PL,
-                                                     float* __restrict__ rhoOut) {
+                                                     float* __restrict__ rhoOut, Lists L,
+                                                     int fresh) {
   const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
   if (a >= n) return;
   const float4 pa = P[a];
@@ -33,102 +235,95 @@ __global__ __launch_bounds__(kThreads) void k_lambda(uint32_t n, PbfConsts c,
     rhoOut[a] = 0.0f;
     return;
   }
-  float gx = 0.0f, gy = 0.0f, gz = 0.0f, rho = 0.0f;
-  float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, crho = 0.0f;
-  forEachNeighbourCell(keys[a], table, c.G, [&](uint32_t s, uint32_t e) {
-    for (uint32_t b = s; b < e; ++b) {
-      const float4 pb = P[b];
-      const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-      const float d2 = dist2(dx, dy, dz);
-      if (d2 <= c.rmax2) {
-        const float r = __builtin_sqrtf(d2);
-        const float w = poly6W(c, r);
-        const float sc = spikyScale(c, r);
-        if (!KAHAN) {
-          gx = __builtin_fmaf(dx * sc, c.rhoRecip, gx);
-          gy = __builtin_fmaf(dy * sc, c.rhoRecip, gy);
-          gz = __builtin_fmaf(dz * sc, c.rhoRecip, gz);
-          rho = __builtin_fmaf(pa.w, w, rho);
-        } else {
-          const float tg[3] = {(dx * sc) * c.rhoRecip, (dy * sc) * c.rhoRecip,
-                               (dz * sc) * c.rhoRecip};
-          float* acc[3] = {&gx, &gy, &gz};
-          float* cmp[3] = {&cgx, &cgy, &cgz};
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            const float sum = *acc[k] + tg[k];
-            *cmp[k] += (fabsf(*acc[k]) >= fabsf(tg[k])) ? (*acc[k] - sum) + tg[k]
-                                                       : (tg[k] - sum) + *acc[k];
-            *acc[k] = sum;
-          }
-          const float term = pa.w * w;
-          const float sum = rho + term;
-          crho += (fabsf(rho) >= fabsf(term)) ? (rho - sum) + term : (term - sum) + rho;
-          rho = sum;
-        }
-      }
-    }
-  });
+  LambdaSums A;
+  forEachAccepted(
+      c, L, 1, fresh != 0, a, pa, keys[a], table, [&](uint32_t b) { return P[b]; },
+      [&](uint32_t, float4 pb, float d2) {
+        lambdaTerm<KAHAN>(c, pa.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, A);
+      });
   if (KAHAN) {
-    gx += cgx;
-    gy += cgy;
-    gz += cgz;
-    rho += crho;
+    A.gx += A.cgx;
+    A.gy += A.cgy;
+    A.gz += A.cgz;
+    A.rho += A.crho;
   }
-  const float norm2 = dist2(gx, gy, gz);
-  const float C1 = rho / c.rho0 - 1.0f;
+  const float norm2 = dist2(A.gx, A.gy, A.gz);
+  const float C1 = A.rho / c.rho0 - 1.0f;
   PL[a] = make_float4(pa.x, pa.y, pa.z, -C1 / (norm2 + c.cfm));
-  rhoOut[a] = rho;
+  rhoOut[a] = A.rho;
+}
+
+// One accepted pair of sph_delta (oclsph_kernel.h:355-358).
+__device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float dx, float dy,
+                                 float dz, float d2, float& ax, float& ay, float& az) {
+  const float r = __builtin_sqrtf(d2);
+  const float q = poly6W(c, r) / c.wdq;
+  const float corr = -c.corrK * ((q * q) * (q * q));
+  const float factor = ((la + lb) + corr) / c.rho0;
+  const float sc = spikyScale(c, r);
+  ax = __builtin_fmaf(dx * sc, factor, ax);
+  ay = __builtin_fmaf(dy * sc, factor, ay);
+  az = __builtin_fmaf(dz * sc, factor, az);
 }
 
 // sph_delta (oclsph_kernel.h:335-384) without colliders; Jacobi: reads PL,
-// writes Pout. deltaP is not stored (nothing reads it after the pass).
+// writes Pout. deltaP is not stored (nothing reads it after the pass). Also
+// raises D to the new displacements (one atomic per wave).
 __global__ __launch_bounds__(kThreads) void k_delta(uint32_t n, PbfConsts c,
                                                     const uint32_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ table,
                                                     const uint8_t* __restrict__ T,
                                                     const float4* __restrict__ PL,
-                                                    const float4* __restrict__ X,
-                                                    float4* __restrict__ Pout) {
+                                                    const float4* __restrict__ X, Lists L,
+                                                    int fresh, float4* __restrict__ Pout) {
   const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
-  if (a >= n) return;
-  const float4 pa = PL[a];
-  const float mass = X[a].w;
-  if (T[a] == LF_OBSTACLE) {
-    Pout[a] = make_float4(pa.x, pa.y, pa.z, mass);
-    return;
-  }
-  const float la = pa.w;
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  forEachNeighbourCell(keys[a], table, c.G, [&](uint32_t s, uint32_t e) {
-    for (uint32_t b = s; b < e; ++b) {
-      const float4 pb = PL[b];
-      const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-      const float d2 = dist2(dx, dy, dz);
-      if (d2 <= c.rmax2) {
-        const float r = __builtin_sqrtf(d2);
-        const float q = poly6W(c, r) / c.wdq;
-        const float corr = -c.corrK * ((q * q) * (q * q));
-        const float factor = ((la + pb.w) + corr) / c.rho0;
-        const float sc = spikyScale(c, r);
-        ax = __builtin_fmaf(dx * sc, factor, ax);
-        ay = __builtin_fmaf(dy * sc, factor, ay);
-        az = __builtin_fmaf(dz * sc, factor, az);
-      }
-    }
-  });
-  // pos = (p* + dp) * scale, clamped to the bounds (:364-371), p* = pos/scale
-  const float acc[3] = {ax, ay, az};
-  const float p[3] = {pa.x, pa.y, pa.z};
-  float o[3];
+  float disp = 0.0f;
+  if (a < n) {
+    const float4 pa = PL[a];
+    const float mass = X[a].w;
+    if (T[a] == LF_OBSTACLE) {
+      Pout[a] = make_float4(pa.x, pa.y, pa.z, mass);
+    } else {
+      const float la = pa.w;
+      float ax = 0.0f, ay = 0.0f, az = 0.0f;
+      forEachAccepted(
+          c, L, 2, fresh != 0, a, pa, keys[a], table, [&](uint32_t b) { return PL[b]; },
+          [&](uint32_t, float4 pb, float d2) {
+            deltaTerm(c, la, pb.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, ax, ay, az);
+          });
+      // pos = (p* + dp) * scale, clamped to the bounds (:364-371), p* = pos/scale
+      const float acc[3] = {ax, ay, az};
+      const float p[3] = {pa.x, pa.y, pa.z};
+      float o[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float pos = (p[k] + acc[k]) * c.scale;
-    pos = (c.minB[k] < pos) ? pos : c.minB[k];
-    pos = (pos < c.maxB[k]) ? pos : c.maxB[k];
-    o[k] = pos / c.scale;
+      for (int k = 0; k < 3; ++k) {
+        float pos = (p[k] + acc[k]) * c.scale;
+        pos = (c.minB[k] < pos) ? pos : c.minB[k];
+        pos = (pos < c.maxB[k]) ? pos : c.maxB[k];
+        o[k] = pos / c.scale;
+      }
+      Pout[a] = make_float4(o[0], o[1], o[2], mass);
+      const float4 p0 = L.P0[a];
+      disp = __builtin_sqrtf(dist2(o[0] - p0.x, o[1] - p0.y, o[2] - p0.z));
+      if (!(disp < 3.0e38f)) disp = 3.0e38f;  // NaN / inf: every list unusable
+    }
   }
-  Pout[a] = make_float4(o[0], o[1], o[2], mass);
+  // per-cell maximum: one atomic per run of equal keys within the wave
+  const uint32_t key = a < n ? keys[a] : 0xFFFFFFFFu;
+  const uint32_t prevKey = __shfl_up(key, 1, 64);
+  float runMax = disp;  // segmented max towards the first lane of each run
+  const bool head = __lane_id() == 0 || prevKey != key;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const float other = __shfl_down(runMax, off, 64);
+    const uint32_t otherKey = __shfl_down(key, off, 64);
+    if (__lane_id() + off < 64 && otherKey == key) runMax = fmaxf(runMax, other);
+  }
+  if (a < n && head && runMax > 0.0f && key + 1u < c.G)
+    atomicMax(&L.cellDisp[key], __float_as_uint(runMax));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
+  if (__lane_id() == 0 && disp > 0.0f) atomicMax(L.maxDisp, __float_as_uint(disp));
 }
 
 // sph_finalise (oclsph_kernel.h:404-416). Also sets P.w to the fluid mass (0
@@ -158,43 +353,6 @@ __global__ __launch_bounds__(kThreads) void k_finalise(uint32_t n, PbfConsts c,
   V[a] = make_float4(__builtin_fmaf(dX, c.invDt, v.x) * c.vd,
                      __builtin_fmaf(dY, c.invDt, v.y) * c.vd,
                      __builtin_fmaf(dZ, c.invDt, v.z) * c.vd, col);
-}
-
-// XSPH viscosity (absent in the reference): v_a += (c/rho0) sum_b m_b (v_b -
-// v_a) W6(r) over fluid b within h, Jacobi (Vin -> Vout). P.w is the fluid
-// mass after finalise.
-__global__ __launch_bounds__(kThreads) void k_xsph(uint32_t n, PbfConsts c,
-                                                   const uint32_t* __restrict__ keys,
-                                                   const uint32_t* __restrict__ table,
-                                                   const uint8_t* __restrict__ T,
-                                                   const float4* __restrict__ P,
-                                                   const float4* __restrict__ Vin,
-                                                   float4* __restrict__ Vout) {
-  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
-  if (a >= n) return;
-  const float4 va = Vin[a];
-  if (T[a] == LF_OBSTACLE) {
-    Vout[a] = va;
-    return;
-  }
-  const float4 pa = P[a];
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  forEachNeighbourCell(keys[a], table, c.G, [&](uint32_t s, uint32_t e) {
-    for (uint32_t b = s; b < e; ++b) {
-      const float4 pb = P[b];
-      const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-      const float d2 = dist2(dx, dy, dz);
-      if (d2 <= c.rmax2 && T[b] != LF_OBSTACLE) {
-        const float mw = pb.w * poly6W(c, __builtin_sqrtf(d2));
-        const float4 vb = Vin[b];
-        ax = __builtin_fmaf(mw, vb.x - va.x, ax);
-        ay = __builtin_fmaf(mw, vb.y - va.y, ay);
-        az = __builtin_fmaf(mw, vb.z - va.z, az);
-      }
-    }
-  });
-  Vout[a] = make_float4(__builtin_fmaf(c.xsphRho, ax, va.x), __builtin_fmaf(c.xsphRho, ay, va.y),
-                        __builtin_fmaf(c.xsphRho, az, va.z), va.w);
 }
 
 __device__ inline void unpackARGB(uint32_t c, float out[4]) {
@@ -251,28 +409,77 @@ __global__ __launch_bounds__(kThreads) void k_diffuse(uint32_t n, PbfConsts c,
   diffused[a] = out;
 }
 
+// XSPH viscosity (absent in the reference): v_a += (c/rho0) sum_b m_b (v_b -
+// v_a) W6(r) over fluid b within h, Jacobi (Vin -> Vout). P.w is the fluid
+// mass after finalise.
+__global__ __launch_bounds__(kThreads) void k_xsph(uint32_t n, PbfConsts c,
+                                                   const uint32_t* __restrict__ keys,
+                                                   const uint32_t* __restrict__ table,
+                                                   const uint8_t* __restrict__ T,
+                                                   const float4* __restrict__ P,
+                                                   const float4* __restrict__ Vin, Lists L,
+                                                   float4* __restrict__ Vout) {
+  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
+  if (a >= n) return;
+  const float4 va = Vin[a];
+  if (T[a] == LF_OBSTACLE) {
+    Vout[a] = va;
+    return;
+  }
+  const float4 pa = P[a];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  forEachAccepted(
+      c, L, 3, false, a, pa, keys[a], table, [&](uint32_t b) { return P[b]; },
+      [&](uint32_t b, float4 pb, float d2) {
+        if (T[b] == LF_OBSTACLE) return;
+        const float mw = pb.w * poly6W(c, __builtin_sqrtf(d2));
+        const float4 vb = Vin[b];
+        ax = __builtin_fmaf(mw, vb.x - va.x, ax);
+        ay = __builtin_fmaf(mw, vb.y - va.y, ay);
+        az = __builtin_fmaf(mw, vb.z - va.z, az);
+      });
+  Vout[a] = make_float4(__builtin_fmaf(c.xsphRho, ax, va.x), __builtin_fmaf(c.xsphRho, ay, va.y),
+                        __builtin_fmaf(c.xsphRho, az, va.z), va.w);
+}
+
+Lists listsOf(Ctx* c) {
+  return Lists{c->nbr.p, c->nbrCount.p, c->P0.p, c->maxDisp.p, c->cellDisp.p};
+}
+
 }  // namespace
 
-int pbfLambda(Ctx* c) {
-  const unsigned nb = blocksFor(c->n, kThreads);
+int pbfBuildLists(Ctx* c) {
   if (c->n == 0) return LF_OK;
+  const unsigned nb = blocksFor(c->n, kThreads);
+  c->listFresh = true;
+  return launch(c, "pbf_build_list", k_build_list, dim3(nb), dim3(kThreads), 0, uint32_t(c->n),
+                c->consts, (const uint32_t*)c->sortKey[c->keysIdx].p, (const uint32_t*)c->table.p,
+                (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, listsOf(c));
+}
+
+int pbfLambda(Ctx* c) {
+  if (c->n == 0) return LF_OK;
+  const unsigned nb = blocksFor(c->n, kThreads);
   const uint32_t* keys = c->sortKey[c->keysIdx].p;
   if (c->consts.flags & LF_FLAG_KAHAN)
     return launch(c, "pbf_lambda_kahan", k_lambda<true>, dim3(nb), dim3(kThreads), 0,
                   uint32_t(c->n), c->consts, keys, (const uint32_t*)c->table.p,
-                  (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p);
+                  (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p,
+                  listsOf(c), int(c->listFresh));
   return launch(c, "pbf_lambda", k_lambda<false>, dim3(nb), dim3(kThreads), 0, uint32_t(c->n),
                 c->consts, keys, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
-                (const float4*)c->P.p, c->PL.p, c->rho.p);
+                (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), int(c->listFresh));
 }
 
 int pbfDelta(Ctx* c) {
   if (c->n == 0) return LF_OK;
   const unsigned nb = blocksFor(c->n, kThreads);
-  return launch(c, "pbf_delta", k_delta, dim3(nb), dim3(kThreads), 0, uint32_t(c->n), c->consts,
+  const int rc = launch(c, "pbf_delta", k_delta, dim3(nb), dim3(kThreads), 0, uint32_t(c->n), c->consts,
                 (const uint32_t*)c->sortKey[c->keysIdx].p, (const uint32_t*)c->table.p,
                 (const uint8_t*)c->T[c->cur].p, (const float4*)c->PL.p,
-                (const float4*)c->X[c->cur].p, c->P.p);
+                (const float4*)c->X[c->cur].p, listsOf(c), int(c->listFresh), c->P.p);
+  c->listFresh = false;  // p* now differs from the predicted positions
+  return rc;
 }
 
 int pbfFinalise(Ctx* c) {
@@ -293,7 +500,7 @@ int pbfXsph(Ctx* c) {
   rc = launch(c, "pbf_xsph", k_xsph, dim3(nb), dim3(kThreads), 0, uint32_t(c->n), c->consts,
               (const uint32_t*)c->sortKey[c->keysIdx].p, (const uint32_t*)c->table.p,
               (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p,
-              (const float4*)c->V[c->cur].p, c->V[nxt].p);
+              (const float4*)c->V[c->cur].p, listsOf(c), c->V[nxt].p);
   if (rc) return rc;
   std::swap(c->V[0], c->V[1]);  // the XSPH output becomes the current V
   return LF_OK;

@@ -549,9 +549,10 @@ template <typename F> class Pbf {
         }
   }
 
-  /* SPH density field rho(x) = sum over fluid b of m_b W6(|x - p*_b|) on the
-   * lattice x_ijk = origin + (i,j,k) * step (solver units). Candidates: the
-   * 27 cells around the sample's own cell, as the PBF passes. */
+  /* SPH density field rho(x) = sum over fluid b of m_b poly6 (h^2 - d^2)^3,
+   * d^2 = |x - p*_b|^2 <= h^2 (W6 written on d^2), on the lattice
+   * x_ijk = origin + (i,j,k) * step (solver units). Candidates: the 27 cells
+   * around the sample's own cell, as the PBF passes; terms added in order. */
   void mcDensityField(const uint32_t S[3], const F origin[3], const F step[3],
                       std::vector<F>& field) const {
     const size_t total = size_t(S[0]) * S[1] * S[2];
@@ -567,9 +568,10 @@ template <typename F> class Pbf {
           forEachCandidate(keyOf(p), [&](uint32_t b) {
             if (ts[b] == LF_OBSTACLE) return;
             const F* pb = &ps[4 * b];
-            const F dx = p[0] - pb[0], dy = p[1] - pb[1], dz = p[2] - pb[2];
-            const F d = std::sqrt(dist2(dx, dy, dz));
-            r = std::fma(pb[3], c.W6(d), r);
+            const F d2 = dist2(p[0] - pb[0], p[1] - pb[1], p[2] - pb[2]);
+            if (!(d2 <= c.hh)) return;
+            const F t = c.hh - d2;
+            r += pb[3] * (c.poly6 * ((t * t) * t));
           });
           field[(size_t(xl) * S[1] + size_t(yl)) * S[2] + z] = r;
         }
