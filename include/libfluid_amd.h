@@ -171,8 +171,15 @@ int lf_mc_sample(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m);
  * oclsph.hpp:461-465, done on the device). */
 int lf_mc_classify_count(lf_ctx* ctx, const lf_mc_params* m);
 /* Triangle emission at scanned offsets, in cube-index order (mc_eval,
- * oclsph_kernel.h:632-735; deterministic instead of atomic_inc). */
+ * oclsph_kernel.h:632-735; deterministic instead of atomic_inc). World
+ * positions use p->scale; p may be NULL to keep the current scale. */
 int lf_mc_emit(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m);
+/* Upload a scalar field on an S[0] x S[1] x S[2] lattice (x-major, sample
+ * (i,j,k) at (origin + (i,j,k) * step) * scale world units) for
+ * classify / emit / download: the GPU counterpart of the CPU mesher
+ * surface::parameterise (surface.hpp:269-384). */
+int lf_mc_field_upload(lf_ctx* ctx, const uint32_t samples[3], const float origin[3],
+                       const float step[3], float scale, const float* field);
 /* Triangle count of the last emit (synchronises). */
 int lf_mc_count(lf_ctx* ctx, size_t* ntri);
 /* Copy up to cap triangles out (synchronises). */
@@ -190,7 +197,10 @@ int lf_mc_set_capacity(lf_ctx* ctx, size_t max_triangles);
  * triangles are written to tris (tris may be NULL when tri_cap is 0). */
 int lf_advance(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m,
                lf_particle* xs, size_t n, lf_mesh_triangle* tris,
-               size_t tri_cap, size_t* ntri);
+               size_t tri_cap, size_t* ntri, uint32_t* order);
+/* (order may be NULL; else order[i] = input index of output particle i.)
+ * When *ntri exceeds tri_cap, the triangles stay on the device and
+ * lf_mc_download fetches them. */
 
 /* ---- inspection (parity tests, teacher forcing) ------------------------- */
 #define LF_ARR_KEYS 1       /* u32[n]   sorted Morton keys */

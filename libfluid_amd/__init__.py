@@ -194,7 +194,24 @@ class Context:
         self._check(self._lib.lf_mc_classify_count(self._h, ctypes.byref(m)))
 
     def mc_emit(self, p, m):
-        self._check(self._lib.lf_mc_emit(self._h, ctypes.byref(p), ctypes.byref(m)))
+        self._check(self._lib.lf_mc_emit(self._h, ctypes.byref(p) if p is not None else None,
+                                         ctypes.byref(m)))
+
+    def mc_field_upload(self, field, origin, step, scale=1.0):
+        """Upload a scalar field (x-major, shape S) for classify / emit."""
+        field = np.ascontiguousarray(field, np.float32)
+        S = (ctypes.c_uint32 * 3)(*field.shape)
+        o = (ctypes.c_float * 3)(*origin)
+        st = (ctypes.c_float * 3)(*step)
+        self._check(self._lib.lf_mc_field_upload(self._h, S, o, st, scale, _ptr(field)))
+
+    def mesh_field(self, field, isolevel, origin, step, scale=1.0):
+        """Marching cubes of an arbitrary scalar field on the GPU."""
+        self.mc_field_upload(field, origin, step, scale)
+        m = mc_density_params(isolevel, tuple(np.asarray(field).shape))
+        self.mc_classify_count(m)
+        self.mc_emit(None, m)
+        return self.mc_triangles()
 
     def mc_count(self):
         n = ctypes.c_size_t()
@@ -398,23 +415,21 @@ class SphSolver:
         lib = self.ctx._lib
         self.ctx.set_wells(config.wells)
         xs = np.ascontiguousarray(xs, dtype=particle_dtype())
-        while True:
-            tris = np.zeros(self._tri_cap, dtype=triangle_dtype())
-            ntri = ctypes.c_size_t()
-            work = xs.copy()
-            rc = lib.lf_advance(self.ctx._h, ctypes.byref(p), ctypes.byref(m), _ptr(work),
-                                len(work), _ptr(tris), self._tri_cap, ctypes.byref(ntri))
-            self.ctx._check(rc)
-            if ntri.value <= self._tri_cap:
-                break
-            self._tri_cap = int(ntri.value)
+        work = xs.copy()
+        order = np.empty(len(work), np.uint32)
+        ntri = ctypes.c_size_t()
+        rc = lib.lf_advance(self.ctx._h, ctypes.byref(p), ctypes.byref(m), _ptr(work),
+                            len(work), None, 0, ctypes.byref(ntri), _ptr(order))
+        self.ctx._check(rc)
         self.ctx.n = len(work)
-        queries = self._queries(config, work) if config.queries else []
-        return work, Result(tris[:ntri.value], queries)
+        tris = self.ctx.mc_triangles()
+        queries = self._queries(config, work, xs["colour"][order]) if config.queries else []
+        return work, Result(tris, queries)
 
-    def _queries(self, config, xs):
+    def _queries(self, config, xs, colours):
         """Spatial queries (oclsph.hpp:770-791): fluid particles of the query
-        point's own cell, their count and mean colour."""
+        point's own cell, their count and mean colour (the colours before this
+        step's diffusion, in this step's Morton order)."""
         G, _, min_ext = self.ctx.grid_info()
         table = self.ctx.read(LF_ARR_TABLE)
         out = []
@@ -425,17 +440,17 @@ class SphSolver:
                      for k in range(3)]
             z = _morton(*cells)
             if z < G and z + 1 < G:
-                sel = xs[table[z]:table[z + 1]]
-                sel = sel[sel["type"] == LF_FLUID]
-                if len(sel):
-                    c = sel["colour"].astype(np.uint64)
+                rng = slice(int(table[z]), int(table[z + 1]))
+                fluid = xs["type"][rng] == LF_FLUID
+                if fluid.any():
+                    c = colours[rng][fluid].astype(np.uint64)
                     argb = np.stack([(c >> 24) & 255, (c >> 16) & 255, (c >> 8) & 255,
                                      c & 255], axis=1).astype(np.float32)
-                    avg = argb.sum(axis=0) / np.float32(len(sel))
+                    avg = argb.sum(axis=0) / np.float32(len(c))
                     packed = 0
                     for ch in avg:
                         packed = (packed << 8) | (int(ch) & 0xFF)
-                    out.append(QueryResult(q.id, tuple(q.point), len(sel), packed))
+                    out.append(QueryResult(q.id, tuple(q.point), len(c), packed))
         return out
 
 

@@ -151,12 +151,13 @@ SIGNATURES = {
     "lf_pbf_step": (_I, [_P, ctypes.POINTER(PbfParams), _U32]),
     "lf_mc_sample": (_I, [_P, ctypes.POINTER(PbfParams), ctypes.POINTER(McParams)]),
     "lf_mc_classify_count": (_I, [_P, ctypes.POINTER(McParams)]),
-    "lf_mc_emit": (_I, [_P, ctypes.POINTER(PbfParams), ctypes.POINTER(McParams)]),
+    "lf_mc_emit": (_I, [_P, _P, ctypes.POINTER(McParams)]),
     "lf_mc_count": (_I, [_P, ctypes.POINTER(_SZ)]),
     "lf_mc_download": (_I, [_P, _P, _SZ, ctypes.POINTER(_SZ)]),
     "lf_mc_set_capacity": (_I, [_P, _SZ]),
     "lf_advance": (_I, [_P, ctypes.POINTER(PbfParams), ctypes.POINTER(McParams), _P, _SZ, _P,
-                        _SZ, ctypes.POINTER(_SZ)]),
+                        _SZ, ctypes.POINTER(_SZ), _P]),
+    "lf_mc_field_upload": (_I, [_P, _P, _P, _P, ctypes.c_float, _P]),
     "lf_read": (_I, [_P, _I, _P, _SZ]),
     "lf_write": (_I, [_P, _I, _P, _SZ]),
     "lf_mc_keep_cases": (_I, [_P, _I]),

@@ -393,8 +393,10 @@ int lf_mc_classify_count(lf_ctx* c, const lf_mc_params* m) {
 
 int lf_mc_emit(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m) {
   if (!c || !m) return LF_ERR_ARG;
-  int rc = lf::setParams(c, p);
-  if (rc) return rc;
+  if (p) {  // NULL keeps the scale of the last step or lf_mc_field_upload
+    const int rc = lf::setParams(c, p);
+    if (rc) return rc;
+  }
   LF_HIP(c, hipSetDevice(c->device));
   return lf::mcEmit(c, *m);
 }
@@ -453,6 +455,30 @@ int lf_mc_download(lf_ctx* c, lf_mesh_triangle* out, size_t cap, size_t* ntri) {
   return rc;
 }
 
+int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[3],
+                       const float step[3], float scale, const float* field) {
+  if (!c || !samples || !origin || !step || !field) return LF_ERR_ARG;
+  for (int k = 0; k < 3; ++k)
+    if (samples[k] < 2) return fail(c, LF_ERR_ARG, "lf_mc_field_upload: samples must be >= 2");
+  if (!(scale > 0.0f)) return fail(c, LF_ERR_ARG, "lf_mc_field_upload: scale must be > 0");
+  const size_t total = size_t(samples[0]) * samples[1] * samples[2];
+  LF_HIP(c, hipSetDevice(c->device));
+  int rc = lf::ensure(c, c->field, total, "mc field");
+  if (rc) return rc;
+  LF_HIP(c, hipMemcpyAsync(c->field.p, field, total * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  LF_HIP(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 3; ++k) {
+    c->S[k] = samples[k];
+    c->mcOrigin[k] = origin[k];
+    c->mcStep[k] = step[k];
+  }
+  c->consts.scale = scale;
+  c->mcAttr = 0;
+  c->mcSampled = true;
+  c->mcCounted = c->mcEmitted = false;
+  return LF_OK;
+}
+
 int lf_mc_set_capacity(lf_ctx* c, size_t max_triangles) {
   if (!c || max_triangles == 0) return LF_ERR_ARG;
   if (max_triangles != c->triCap) {
@@ -465,7 +491,8 @@ int lf_mc_set_capacity(lf_ctx* c, size_t max_triangles) {
 }
 
 int lf_advance(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m, lf_particle* xs,
-               size_t n, lf_mesh_triangle* tris, size_t tri_cap, size_t* ntri) {
+               size_t n, lf_mesh_triangle* tris, size_t tri_cap, size_t* ntri,
+               uint32_t* orderOut) {
   if (!c || !p) return LF_ERR_ARG;
   if (ntri) *ntri = 0;
   if (n == 0) return LF_OK;  // oclsph.hpp:711-715: empty input, empty result
@@ -497,7 +524,7 @@ int lf_advance(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m, lf_part
     }
     if ((rc = lf_mc_emit(c, &q, m))) return rc;
     size_t got = 0;
-    if (tris && tri_cap) {
+    if (tris && tri_cap >= count) {
       if ((rc = lf_mc_download(c, tris, tri_cap, &got))) return rc;
     }
   }
@@ -506,6 +533,7 @@ int lf_advance(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m, lf_part
   if ((rc = lf_state_download(c, n, pos.data(), vel.data(), mass.data(), type.data(),
                               colour.data(), order.data())))
     return rc;
+  if (orderOut) std::memcpy(orderOut, order.data(), n * sizeof(uint32_t));
   // oclsph.hpp:874-892: xs leaves in this step's Morton order
   for (size_t i = 0; i < n; ++i) {
     lf_particle& o = xs[i];
