@@ -379,3 +379,18 @@ def test_cpp_dropin_solver(ctx, tmp_path):
     _eq(tris["c"], res.triangles["c"], "C++ vs Python mesh colours")
     assert qres == [(q.id, q.neighbours, q.average_colour) for q in res.queries]
     assert len(qres) >= 2
+
+
+def test_mesh_arbitrary_field(ctx):
+    """lf_mc_field_upload + classify/emit on a sphere field: the GPU mesher of
+    an arbitrary scalar field against the oracle's marching cubes."""
+    S, r = 57, 0.3
+    step = np.float32(1.0) / np.float32(S - 1)
+    ax = np.arange(S, dtype=np.float32) * step
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    field = (np.float32(r) - np.sqrt(((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
+                                     .astype(np.float32))).astype(np.float32)
+    tris = ctx.mesh_field(field, 0.0, (0.0, 0.0, 0.0), (float(step),) * 3, 2.0)
+    otris = oracle.mc_emit(field.astype(np.float64), 0.0, (0, 0, 0), (float(step),) * 3, 2.0)
+    assert len(tris) == len(otris) > 1000
+    _eq(tris["v"], otris["v"], "sphere mesh")
