@@ -202,10 +202,63 @@ int lf_advance(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m,
  * When *ntri exceeds tri_cap, the triangles stay on the device and
  * lf_mc_download fetches them. */
 
+/* ---- batches: many independent scenes per launch -------------------------
+ * No reference counterpart: libfluid steps one scene per solver. Here the
+ * scenes of a batch share every launch of a step. Scene s owns the particle
+ * range [offsets[s], offsets[s+1]) (offsets = prefix sums of counts) in both
+ * upload and sorted order, its own lf_pbf_params (bounds, scale, gravity,
+ * xsph_c, ...) and its own slice of the cell table, so each scene's results
+ * are those of stepping it alone. All scenes must share h, iterations and
+ * flags (they select the launch sequence). The one difference from stepping a
+ * scene alone is the position in the sorted order of particles whose cell
+ * lies outside the scene's table (there are none while particles stay within
+ * 2h of the bounds): they keep upload order instead of key order. */
+int lf_batch_upload(lf_ctx* ctx, uint32_t n_scenes, const lf_pbf_params* params,
+                    const size_t* counts, const float* pos, const float* vel,
+                    const float* mass, const int32_t* type, const uint32_t* colour);
+/* nsteps steps of every scene of the batch. */
+int lf_batch_step(lf_ctx* ctx, uint32_t nsteps);
+/* Scene count and offsets (n_scenes + 1 entries; offsets may be NULL). */
+int lf_batch_info(const lf_ctx* ctx, uint32_t* n_scenes, size_t* offsets);
+/* Scene of the following lf_mc_sample (params NULL) and lf_grid_info calls. */
+int lf_batch_select(lf_ctx* ctx, uint32_t scene);
+
+/* ---- execution configuration ----------------------------------------------
+ * threads: workgroup size of the particle kernels (128, 256 or 512; each
+ * workgroup works on chunks of at most that many particles of one scene).
+ * persistent: 1 = the pair kernels (list build, lambda, delta, XSPH) run a
+ * device-sized grid that takes chunks from a work queue; 0 = one workgroup per
+ * chunk. skin: neighbour-list skin as a fraction of h (0 turns the lists'
+ * reuse off: every pass after the first enumerates cells). None of these
+ * change results, only speed. */
+int lf_exec_config(lf_ctx* ctx, uint32_t threads, int persistent, float skin);
+typedef struct lf_tune_result {
+  uint32_t threads;
+  int32_t persistent;
+  float skin;
+  float ms_per_step;   /* best configuration's mean step time */
+  uint32_t trials;     /* configurations timed */
+} lf_tune_result;
+/* Times `steps` steps (plus one warm-up) of the current state under each
+ * configuration of threads x persistent x skin and keeps the fastest. The
+ * state is restored after every trial: the call does not advance the
+ * simulation. */
+int lf_autotune(lf_ctx* ctx, uint32_t steps, lf_tune_result* out);
+
+/* ---- HIP graphs --------------------------------------------------------------
+ * Captures nsteps (even, so the double buffers return to their places) whole
+ * steps, each followed by the marching cubes of m when m is not NULL (sample,
+ * classify + count, emit), into one HIP graph; lf_graph_launch replays it
+ * `times` times. Needs one eager step with the same sizes first (buffers are
+ * not allocated during capture). Uploads and parameter changes drop the
+ * graph. */
+int lf_graph_capture(lf_ctx* ctx, uint32_t nsteps, const lf_mc_params* m);
+int lf_graph_launch(lf_ctx* ctx, uint32_t times);
+
 /* ---- inspection (parity tests, teacher forcing) ------------------------- */
-#define LF_ARR_KEYS 1       /* u32[n]   sorted Morton keys */
+#define LF_ARR_KEYS 1       /* u32[n]   sorted keys (scene table base + Morton key) */
 #define LF_ARR_ORDER 2      /* u32[n]   upload index of each sorted particle */
-#define LF_ARR_TABLE 3      /* u32[G]   dense cell-start table */
+#define LF_ARR_TABLE 3      /* u32[G]   dense cell-start table (all scenes' tables) */
 #define LF_ARR_PSTAR 4      /* f32[n*4] p* (x,y,z,mass) in sorted order */
 #define LF_ARR_LAMBDA 5     /* f32[n*4] (p*, lambda) of the last lambda pass */
 #define LF_ARR_POSITION 6   /* f32[n*4] world position (x,y,z,mass) */
@@ -220,6 +273,7 @@ int lf_advance(lf_ctx* ctx, const lf_pbf_params* p, const lf_mc_params* m,
 #define LF_ARR_LISTSTATS 15 /* u32[4]   neighbour lists: [0] float bits of the largest
                                displacement since the build, [1..3] particles that
                                enumerated cells instead (lambda, delta, XSPH) */
+#define LF_ARR_CELLKEY 16   /* u32[n]   scene-local Morton cell key of each sorted particle */
 int lf_read(lf_ctx* ctx, int array, void* dst, size_t bytes);
 /* Keep the case index of every cube for lf_read(LF_ARR_CASES) (off by
  * default: 1 byte per cube of extra traffic). */

@@ -186,9 +186,65 @@ class Context:
     def sync(self):
         self._check(self._lib.lf_sync(self._h))
 
+    # -- batches, execution configuration, graphs ---------------------------
+    def batch_upload(self, params, scenes):
+        """Several scenes in one context. params: one PbfParams per scene;
+        scenes: dicts with position, velocity, mass[, type, colour]."""
+        counts = np.array([len(np.asarray(sc["position"]).reshape(-1, 3)) for sc in scenes],
+                          dtype=np.uint64)
+        n = int(counts.sum())
+        pos = np.concatenate([np.asarray(sc["position"], np.float32).reshape(-1, 3)
+                              for sc in scenes]) if n else np.zeros((0, 3), np.float32)
+        vel = np.concatenate([np.asarray(sc["velocity"], np.float32).reshape(-1, 3)
+                              for sc in scenes]) if n else np.zeros((0, 3), np.float32)
+        mass = np.concatenate([np.broadcast_to(np.asarray(sc["mass"], np.float32),
+                                               (int(c),)) for sc, c in zip(scenes, counts)])
+        typ = np.concatenate([np.asarray(sc.get("type", np.zeros(int(c), np.int32)), np.int32)
+                              for sc, c in zip(scenes, counts)])
+        col = np.concatenate([np.asarray(sc.get("colour", np.full(int(c), 0xFF2D1010)),
+                                         np.uint32) for sc, c in zip(scenes, counts)])
+        arr = (_abi.PbfParams * len(params))(*params)
+        pos, vel, mass, typ, col = (np.ascontiguousarray(a) for a in (pos, vel, mass, typ, col))
+        self._check(self._lib.lf_batch_upload(
+            self._h, len(params), ctypes.cast(arr, ctypes.c_void_p), _ptr(counts), _ptr(pos),
+            _ptr(vel), _ptr(mass), _ptr(typ), _ptr(col)))
+        self.n = n
+
+    def batch_step(self, nsteps=1):
+        self._check(self._lib.lf_batch_step(self._h, nsteps))
+
+    def batch_info(self):
+        """Scene offsets (n_scenes + 1 entries)."""
+        S = ctypes.c_uint32()
+        self._check(self._lib.lf_batch_info(self._h, ctypes.byref(S), None))
+        off = np.zeros(S.value + 1, np.uint64)
+        self._check(self._lib.lf_batch_info(self._h, ctypes.byref(S), _ptr(off)))
+        return off.astype(np.int64)
+
+    def batch_select(self, scene):
+        self._check(self._lib.lf_batch_select(self._h, scene))
+
+    def exec_config(self, threads=256, persistent=False, skin=0.2):
+        self._check(self._lib.lf_exec_config(self._h, threads, 1 if persistent else 0, skin))
+
+    def autotune(self, steps=3):
+        r = _abi.TuneResult()
+        self._check(self._lib.lf_autotune(self._h, steps, ctypes.byref(r)))
+        return dict(threads=r.threads, persistent=bool(r.persistent), skin=r.skin,
+                    ms_per_step=r.ms_per_step, trials=r.trials)
+
+    def graph_capture(self, nsteps=2, m=None):
+        self._check(self._lib.lf_graph_capture(self._h, nsteps,
+                                               ctypes.byref(m) if m is not None else None))
+
+    def graph_launch(self, times=1):
+        self._check(self._lib.lf_graph_launch(self._h, times))
+
     # -- marching cubes -----------------------------------------------------
     def mc_sample(self, p, m):
-        self._check(self._lib.lf_mc_sample(self._h, ctypes.byref(p), ctypes.byref(m)))
+        """p None: the constants of the last step (a batch: the selected scene's)."""
+        self._check(self._lib.lf_mc_sample(self._h, ctypes.byref(p) if p is not None else None,
+                                           ctypes.byref(m)))
 
     def mc_classify_count(self, m):
         self._check(self._lib.lf_mc_classify_count(self._h, ctypes.byref(m)))
@@ -260,7 +316,9 @@ class Context:
         if array in (LF_ARR_KEYS, LF_ARR_ORDER, LF_ARR_DIFFUSED):
             out = np.empty(n, np.uint32)
         elif array == LF_ARR_TABLE:
-            out = np.empty(self.grid_info()[0], np.uint32)
+            out = np.empty(self.grid_info()[0] if count is None else count, np.uint32)
+        elif array == _abi.LF_ARR_CELLKEY:
+            out = np.empty(n, np.uint32)
         elif array in (LF_ARR_PSTAR, LF_ARR_LAMBDA, LF_ARR_POSITION, LF_ARR_VELOCITY):
             out = np.empty((n, 4), np.float32)
         elif array == LF_ARR_TYPE:

@@ -42,6 +42,7 @@ LF_ARR_MCCOLOUR = 12
 LF_ARR_CASES = 13
 LF_ARR_DENSITY = 14
 LF_ARR_LISTSTATS = 15
+LF_ARR_CELLKEY = 16
 
 _f3 = ctypes.c_float * 3
 _u3 = ctypes.c_uint32 * 3
@@ -105,6 +106,17 @@ class MeshTriangle(ctypes.Structure):
     ]
 
 
+class TuneResult(ctypes.Structure):
+    """lf_tune_result: the configuration lf_autotune kept."""
+    _fields_ = [
+        ("threads", ctypes.c_uint32),
+        ("persistent", ctypes.c_int32),
+        ("skin", ctypes.c_float),
+        ("ms_per_step", ctypes.c_float),
+        ("trials", ctypes.c_uint32),
+    ]
+
+
 # numpy views of the same records
 def particle_dtype():
     import numpy as np
@@ -149,7 +161,7 @@ SIGNATURES = {
     "lf_pbf_finalise": (_I, [_P, ctypes.POINTER(PbfParams)]),
     "lf_pbf_xsph": (_I, [_P, ctypes.POINTER(PbfParams)]),
     "lf_pbf_step": (_I, [_P, ctypes.POINTER(PbfParams), _U32]),
-    "lf_mc_sample": (_I, [_P, ctypes.POINTER(PbfParams), ctypes.POINTER(McParams)]),
+    "lf_mc_sample": (_I, [_P, _P, ctypes.POINTER(McParams)]),
     "lf_mc_classify_count": (_I, [_P, ctypes.POINTER(McParams)]),
     "lf_mc_emit": (_I, [_P, _P, ctypes.POINTER(McParams)]),
     "lf_mc_count": (_I, [_P, ctypes.POINTER(_SZ)]),
@@ -167,6 +179,14 @@ SIGNATURES = {
     "lf_profile_reset": (_I, [_P]),
     "lf_kernel_stats": (_I, [_P, ctypes.c_char_p, _SZ]),
     "lf_copy_probe": (_I, [_P, _SZ, _I, ctypes.POINTER(ctypes.c_float)]),
+    "lf_batch_upload": (_I, [_P, _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "lf_batch_step": (_I, [_P, _U32]),
+    "lf_batch_info": (_I, [_P, ctypes.POINTER(_U32), _P]),
+    "lf_batch_select": (_I, [_P, _U32]),
+    "lf_exec_config": (_I, [_P, _U32, _I, ctypes.c_float]),
+    "lf_autotune": (_I, [_P, _U32, ctypes.POINTER(TuneResult)]),
+    "lf_graph_capture": (_I, [_P, _U32, _P]),
+    "lf_graph_launch": (_I, [_P, _U32]),
 }
 
 _lib = None

@@ -17,6 +17,12 @@ int fail(Ctx* c, int code, const std::string& what) {
   return code;
 }
 
+void dropGraph(Ctx* c) {
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+  c->graphSteps = 0;
+}
+
 int hipFail(Ctx* c, hipError_t e, const char* where) {
   if (c) c->err = std::string(where) + ": " + hipGetErrorString(e);
   return LF_ERR_HIP;
@@ -128,11 +134,98 @@ static int setParams(Ctx* c, const lf_pbf_params* p) {
   return LF_OK;
 }
 
+// Per-scene constants and the chunk list on the device. A single-scene
+// context is a batch of one with keyBase 0 and exact keys (the reference's
+// sort order); a batch of several gives scene s the key range [keyBase_s,
+// keyBase_s + G_s] with out-of-table keys clamped to G_s. Uploads happen only
+// when something changed, never during graph capture.
+int prepareScenes(Ctx* c) {
+  std::vector<PbfConsts> consts;
+  std::vector<uint32_t> begin;
+  uint64_t total = 0;
+  if (!c->batch) {
+    PbfConsts k = c->consts;
+    k.keyBase = 0;
+    k.clampKeys = 0;
+    consts.push_back(k);
+    begin = {0u, uint32_t(c->n)};
+    total = k.G;
+  } else {
+    const size_t S = c->sceneParams.size();
+    consts.resize(S);
+    for (size_t s = 0; s < S; ++s) {
+      std::string err;
+      const int rc = foldConsts(c->sceneParams[s], c->skinFrac, consts[s], err);
+      if (rc) return fail(c, rc, "scene " + std::to_string(s) + ": " + err);
+      consts[s].keyBase = uint32_t(total);
+      consts[s].clampKeys = S > 1 ? 1u : 0u;
+      total += uint64_t(consts[s].G) + (S > 1 ? 1u : 0u);
+    }
+    begin = c->sceneBegin;
+    if (total >= (uint64_t(1) << 30))
+      return fail(c, LF_ERR_MORTON, "batch: the scenes' cell tables exceed 2^30 entries in total");
+  }
+  const bool sameConsts = consts.size() == c->uploadedConsts.size() &&
+                          std::memcmp(consts.data(), c->uploadedConsts.data(),
+                                      consts.size() * sizeof(PbfConsts)) == 0;
+  const bool sameChunks = begin == c->uploadedBegin && c->chunksBuiltFor == c->chunkThreads;
+  c->sceneConsts = consts;
+  c->tableTotal = uint32_t(total);
+  if (!c->batch) c->sceneBegin = begin;
+  if (sameConsts && sameChunks && c->dConsts.p && c->chunks.p) return LF_OK;
+  if (c->capturing)
+    return fail(c, LF_ERR_STATE, "parameters or sizes changed during graph capture");
+  int rc;
+  dropGraph(c);
+  if (!sameConsts) {
+    if ((rc = ensure(c, c->dConsts, consts.size(), "scene constants"))) return rc;
+    LF_HIP(c, hipMemcpyAsync(c->dConsts.p, consts.data(), consts.size() * sizeof(PbfConsts),
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<Chunk> chunks;
+  if (!sameChunks) {
+    const uint32_t B = c->chunkThreads;
+    for (size_t s = 0; s + 1 < begin.size(); ++s)
+      for (uint32_t a = begin[s]; a < begin[s + 1]; a += B)
+        chunks.push_back(Chunk{uint32_t(s), a, std::min(a + B, begin[s + 1]), 0u});
+    if ((rc = ensure(c, c->chunks, std::max<size_t>(chunks.size(), 1), "chunks"))) return rc;
+    if (!chunks.empty())
+      LF_HIP(c, hipMemcpyAsync(c->chunks.p, chunks.data(), chunks.size() * sizeof(Chunk),
+                               hipMemcpyHostToDevice, c->stream));
+    c->nChunks = uint32_t(chunks.size());
+  }
+  if (c->persistent && !c->queues.p) {
+    if ((rc = ensure(c, c->queues, 2 * kQueueSlots, "work queues"))) return rc;
+    LF_HIP(c, hipMemsetAsync(c->queues.p, 0, c->queues.cap * sizeof(uint32_t), c->stream));
+  }
+  LF_HIP(c, hipStreamSynchronize(c->stream));  // the host vectors above die at return
+  c->uploadedConsts = consts;
+  c->uploadedBegin = begin;
+  c->chunksBuiltFor = c->chunkThreads;
+  return LF_OK;
+}
+
+Work workOf(Ctx* c, int queueSlot) {
+  Work W{};
+  W.consts = c->dConsts.p;
+  W.chunks = c->chunks.p;
+  W.nChunks = c->nChunks;
+  W.queue = (c->persistent && queueSlot >= 0 && queueSlot < kQueueSlots && c->queues.p)
+                ? c->queues.p + 2 * queueSlot
+                : nullptr;
+  return W;
+}
+
 static int requireGrid(Ctx* c, const char* who) {
   if (!c->gridValid) return fail(c, LF_ERR_STATE, std::string(who) + ": call lf_grid_build first");
   if (c->consts.G != c->gridG)
     return fail(c, LF_ERR_STATE, std::string(who) + ": parameters change the grid; rebuild it");
   return LF_OK;
+}
+
+static int batchOnly(Ctx* c, const char* who) {
+  return fail(c, LF_ERR_STATE,
+              std::string(who) + ": the context holds a batch (use lf_batch_step / lf_batch_select)");
 }
 
 static int pbfStepOnce(Ctx* c) {
@@ -187,6 +280,10 @@ int lf_ctx_create(int device, lf_ctx** out) {
   lf_ctx* c = new (std::nothrow) lf_ctx();
   if (!c) return LF_ERR_ARG;
   c->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      cus > 0)
+    c->persistentBlocks = uint32_t(cus) * 8u;  // 8 workgroups of 256 lanes fill a CU's 32 waves
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return LF_ERR_HIP;
@@ -233,6 +330,11 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->triPos);
   lf::release(c->triNrm);
   lf::release(c->triCol);
+  lf::release(c->dConsts);
+  lf::release(c->chunks);
+  lf::release(c->queues);
+  lf::release(c->localKey);
+  lf::dropGraph(c);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return LF_OK;
@@ -281,9 +383,14 @@ int lf_state_upload(lf_ctx* c, size_t n, const float* pos, const float* vel, con
     LF_HIP(c, hipStreamSynchronize(c->stream));
   }
   c->n = n;
+  c->batch = false;
+  c->sceneParams.clear();
+  c->sceneBegin = {0u, uint32_t(n)};
+  c->mcScene = 0;
   c->gridValid = false;
   c->stepDone = false;
   c->mcSampled = c->mcCounted = c->mcEmitted = false;
+  lf::dropGraph(c);
   return LF_OK;
 }
 
@@ -343,6 +450,7 @@ int lf_set_wells(lf_ctx* c, size_t n, const lf_well* wells) {
 
 int lf_grid_build(lf_ctx* c, const lf_pbf_params* p) {
   if (!c) return LF_ERR_ARG;
+  if (c->batch) return lf::batchOnly(c, "lf_grid_build");
   int rc = lf::setParams(c, p);
   if (rc) return rc;
   c->params = *p;
@@ -354,10 +462,12 @@ int lf_grid_build(lf_ctx* c, const lf_pbf_params* p) {
 
 #define LF_STAGE(fn, who)                                  \
   if (!c) return LF_ERR_ARG;                               \
+  if (c->batch) return lf::batchOnly(c, who);              \
   int rc = lf::setParams(c, p);                            \
   if (rc) return rc;                                       \
   if ((rc = lf::requireGrid(c, who))) return rc;           \
   LF_HIP(c, hipSetDevice(c->device));                      \
+  if ((rc = lf::prepareScenes(c))) return rc;              \
   return fn(c);
 
 int lf_pbf_diffuse(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(lf::pbfDiffuse, "lf_pbf_diffuse") }
@@ -368,6 +478,7 @@ int lf_pbf_xsph(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(lf::pbfXsph, "lf_p
 
 int lf_pbf_step(lf_ctx* c, const lf_pbf_params* p, uint32_t nsteps) {
   if (!c) return LF_ERR_ARG;
+  if (c->batch) return lf::batchOnly(c, "lf_pbf_step");
   int rc = lf::setParams(c, p);
   if (rc) return rc;
   c->params = *p;
@@ -379,8 +490,11 @@ int lf_pbf_step(lf_ctx* c, const lf_pbf_params* p, uint32_t nsteps) {
 
 int lf_mc_sample(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m) {
   if (!c || !m) return LF_ERR_ARG;
-  int rc = lf::setParams(c, p);
-  if (rc) return rc;
+  if (p) {  // NULL: the constants of the last step (in a batch: the selected scene's)
+    if (c->batch) return fail(c, LF_ERR_ARG, "lf_mc_sample: pass NULL params for a batch scene");
+    const int rc = lf::setParams(c, p);
+    if (rc) return rc;
+  }
   LF_HIP(c, hipSetDevice(c->device));
   return lf::mcSample(c, *m);
 }
@@ -393,9 +507,9 @@ int lf_mc_classify_count(lf_ctx* c, const lf_mc_params* m) {
 
 int lf_mc_emit(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m) {
   if (!c || !m) return LF_ERR_ARG;
-  if (p) {  // NULL keeps the scale of the last step or lf_mc_field_upload
-    const int rc = lf::setParams(c, p);
-    if (rc) return rc;
+  if (p) {  // NULL keeps the scale of the last lf_mc_sample or lf_mc_field_upload
+    if (!(p->scale > 0.0f)) return fail(c, LF_ERR_ARG, "lf_mc_emit: scale must be > 0");
+    c->mcScale = p->scale;
   }
   LF_HIP(c, hipSetDevice(c->device));
   return lf::mcEmit(c, *m);
@@ -472,7 +586,7 @@ int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[
     c->mcOrigin[k] = origin[k];
     c->mcStep[k] = step[k];
   }
-  c->consts.scale = scale;
+  c->mcScale = scale;
   c->mcAttr = 0;
   c->mcSampled = true;
   c->mcCounted = c->mcEmitted = false;
@@ -558,7 +672,8 @@ int lf_read(lf_ctx* c, int array, void* dst, size_t bytes) {
   switch (array) {
     case LF_ARR_KEYS: src = c->sortKey[c->keysIdx].p; avail = n * 4; break;
     case LF_ARR_ORDER: src = c->sortVal[c->keysIdx].p; avail = n * 4; break;
-    case LF_ARR_TABLE: src = c->table.p; avail = size_t(c->consts.G) * 4; break;
+    case LF_ARR_TABLE: src = c->table.p; avail = size_t(c->gridTotal) * 4; break;
+    case LF_ARR_CELLKEY: src = c->localKey.p; avail = n * 4; break;
     case LF_ARR_PSTAR: src = c->P.p; avail = n * 16; break;
     case LF_ARR_LAMBDA: src = c->PL.p; avail = n * 16; break;
     case LF_ARR_POSITION: src = c->X[c->cur].p; avail = n * 16; break;
@@ -615,10 +730,12 @@ int lf_write(lf_ctx* c, int array, const void* src, size_t bytes) {
 
 int lf_grid_info(const lf_ctx* c, uint32_t* table_size, uint32_t extent[3], float min_extent[3]) {
   if (!c) return LF_ERR_ARG;
-  if (table_size) *table_size = c->consts.G;
-  for (int k = 0; k < 3; ++k) {
-    if (extent) extent[k] = c->consts.extent[k];
-    if (min_extent) min_extent[k] = c->consts.minExt[k];
+  const lf::PbfConsts& k =
+      (c->batch && size_t(c->mcScene) < c->sceneConsts.size()) ? c->sceneConsts[c->mcScene] : c->consts;
+  if (table_size) *table_size = k.G;
+  for (int i = 0; i < 3; ++i) {
+    if (extent) extent[i] = k.extent[i];
+    if (min_extent) min_extent[i] = k.minExt[i];
   }
   return LF_OK;
 }
@@ -665,6 +782,232 @@ int lf_copy_probe(lf_ctx* c, size_t bytes, int reps, float* ms) {
   if (!c || !ms || bytes < 16 || reps < 1) return LF_ERR_ARG;
   LF_HIP(c, hipSetDevice(c->device));
   return lf::copyProbe(c, bytes, reps, ms);
+}
+
+int lf_batch_upload(lf_ctx* c, uint32_t nScenes, const lf_pbf_params* params,
+                    const size_t* counts, const float* pos, const float* vel, const float* mass,
+                    const int32_t* type, const uint32_t* colour) {
+  if (!c) return LF_ERR_ARG;
+  if (nScenes == 0 || !params || !counts) return fail(c, LF_ERR_ARG, "lf_batch_upload: no scenes");
+  size_t n = 0;
+  std::vector<uint32_t> begin(nScenes + 1, 0u);
+  for (uint32_t s = 0; s < nScenes; ++s) {
+    n += counts[s];
+    if (n >= (size_t(1) << 31)) return fail(c, LF_ERR_ARG, "lf_batch_upload: too many particles");
+    begin[s + 1] = uint32_t(n);
+    const lf_pbf_params& a = params[0];
+    const lf_pbf_params& b = params[s];
+    if (b.h != a.h || b.iterations != a.iterations || b.flags != a.flags)
+      return fail(c, LF_ERR_ARG, "lf_batch_upload: scenes must share h, iterations and flags");
+    lf::PbfConsts k{};
+    std::string err;
+    const int rc = lf::foldConsts(b, c->skinFrac, k, err);
+    if (rc) return fail(c, rc, "scene " + std::to_string(s) + ": " + err);
+  }
+  int rc = lf_state_upload(c, n, pos, vel, mass, type, colour);
+  if (rc) return rc;
+  c->batch = true;
+  c->sceneParams.assign(params, params + nScenes);
+  c->sceneBegin = begin;
+  c->params = params[0];
+  if ((rc = lf::setParams(c, &params[0]))) return rc;
+  c->mcScene = 0;
+  return LF_OK;
+}
+
+int lf_batch_step(lf_ctx* c, uint32_t nsteps) {
+  if (!c) return LF_ERR_ARG;
+  if (!c->batch) return fail(c, LF_ERR_STATE, "lf_batch_step: no batch (call lf_batch_upload)");
+  LF_HIP(c, hipSetDevice(c->device));
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    const int rc = lf::pbfStepOnce(c);
+    if (rc) return rc;
+  }
+  return LF_OK;
+}
+
+int lf_batch_info(const lf_ctx* c, uint32_t* nScenes, size_t* offsets) {
+  if (!c) return LF_ERR_ARG;
+  const uint32_t S = c->batch ? uint32_t(c->sceneParams.size()) : 1u;
+  if (nScenes) *nScenes = S;
+  if (offsets) {
+    if (c->batch)
+      for (uint32_t s = 0; s <= S; ++s) offsets[s] = c->sceneBegin[s];
+    else {
+      offsets[0] = 0;
+      offsets[1] = c->n;
+    }
+  }
+  return LF_OK;
+}
+
+int lf_batch_select(lf_ctx* c, uint32_t scene) {
+  if (!c) return LF_ERR_ARG;
+  const uint32_t S = c->batch ? uint32_t(c->sceneParams.size()) : 1u;
+  if (scene >= S) return fail(c, LF_ERR_ARG, "lf_batch_select: no such scene");
+  c->mcScene = int(scene);
+  return LF_OK;
+}
+
+int lf_exec_config(lf_ctx* c, uint32_t threads, int persistent, float skin) {
+  if (!c) return LF_ERR_ARG;
+  if (threads != 128 && threads != 256 && threads != 512)
+    return fail(c, LF_ERR_ARG, "lf_exec_config: threads must be 128, 256 or 512");
+  if (!(skin >= 0.0f && skin <= 1.0f))
+    return fail(c, LF_ERR_ARG, "lf_exec_config: skin must be in [0, 1] (fraction of h)");
+  lf::dropGraph(c);
+  c->chunkThreads = threads;
+  c->persistent = persistent != 0;
+  c->skinFrac = skin;
+  c->gridValid = false;  // constants fold the skin: the next call must step or build the grid
+  if (!c->batch && c->params.h > 0.0f) (void)lf::setParams(c, &c->params);
+  return LF_OK;
+}
+
+int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
+  if (!c || steps == 0) return LF_ERR_ARG;
+  if (c->n == 0 || !(c->params.h > 0.0f))
+    return fail(c, LF_ERR_STATE, "lf_autotune: needs a state and one step (for its parameters)");
+  LF_HIP(c, hipSetDevice(c->device));
+  const size_t n = c->n;
+  const int cur0 = c->cur;
+  // snapshot of the state
+  float4 *x0 = nullptr, *v0 = nullptr;
+  uint8_t* t0 = nullptr;
+  uint32_t* id0 = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LF_OK;
+  auto cleanup = [&]() {
+    if (x0) (void)hipFree(x0);
+    if (v0) (void)hipFree(v0);
+    if (t0) (void)hipFree(t0);
+    if (id0) (void)hipFree(id0);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  auto hipOk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == LF_OK) rc = lf::hipFail(c, e, what);
+    return rc == LF_OK;
+  };
+  const bool ok = hipOk(hipMalloc(reinterpret_cast<void**>(&x0), n * sizeof(float4)), "autotune") &&
+                  hipOk(hipMalloc(reinterpret_cast<void**>(&v0), n * sizeof(float4)), "autotune") &&
+                  hipOk(hipMalloc(reinterpret_cast<void**>(&t0), n), "autotune") &&
+                  hipOk(hipMalloc(reinterpret_cast<void**>(&id0), n * sizeof(uint32_t)), "autotune") &&
+                  hipOk(hipEventCreate(&e0), "autotune") && hipOk(hipEventCreate(&e1), "autotune");
+  if (!ok) {
+    cleanup();
+    return rc;
+  }
+  auto copyState = [&](bool save) {
+    const size_t b4 = n * sizeof(float4);
+    if (save) {
+      return hipOk(hipMemcpyAsync(x0, c->X[cur0].p, b4, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
+             hipOk(hipMemcpyAsync(v0, c->V[cur0].p, b4, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
+             hipOk(hipMemcpyAsync(t0, c->T[cur0].p, n, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
+             hipOk(hipMemcpyAsync(id0, c->ID[cur0].p, n * 4, hipMemcpyDeviceToDevice, c->stream), "autotune");
+    }
+    c->cur = cur0;
+    return hipOk(hipMemcpyAsync(c->X[cur0].p, x0, b4, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
+           hipOk(hipMemcpyAsync(c->V[cur0].p, v0, b4, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
+           hipOk(hipMemcpyAsync(c->T[cur0].p, t0, n, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
+           hipOk(hipMemcpyAsync(c->ID[cur0].p, id0, n * 4, hipMemcpyDeviceToDevice, c->stream), "autotune");
+  };
+  if (!copyState(true)) {
+    cleanup();
+    return rc;
+  }
+  const bool prof = c->prof;
+  c->prof = false;
+  const uint32_t threadsList[3] = {128, 256, 512};
+  const float skins[3] = {0.1f, 0.2f, 0.3f};
+  lf_tune_result best{};
+  best.ms_per_step = INFINITY;
+  uint32_t trials = 0;
+  for (uint32_t th : threadsList)
+    for (int pers = 0; pers < 2 && rc == LF_OK; ++pers)
+      for (float sk : skins) {
+        if ((rc = lf_exec_config(c, th, pers, sk))) break;
+        if (!copyState(false)) break;
+        if ((rc = lf::pbfStepOnce(c))) break;  // warm-up (allocations, caches)
+        if (!hipOk(hipEventRecord(e0, c->stream), "autotune")) break;
+        for (uint32_t s = 0; s < steps && rc == LF_OK; ++s) rc = lf::pbfStepOnce(c);
+        if (rc) break;
+        if (!hipOk(hipEventRecord(e1, c->stream), "autotune") ||
+            !hipOk(hipEventSynchronize(e1), "autotune"))
+          break;
+        float ms = 0.0f;
+        if (!hipOk(hipEventElapsedTime(&ms, e0, e1), "autotune")) break;
+        ++trials;
+        if (ms / float(steps) < best.ms_per_step) {
+          best.threads = th;
+          best.persistent = pers;
+          best.skin = sk;
+          best.ms_per_step = ms / float(steps);
+        }
+      }
+  c->prof = prof;
+  if (rc == LF_OK) rc = lf_exec_config(c, best.threads, best.persistent, best.skin);
+  if (rc == LF_OK) (void)copyState(false);
+  if (rc == LF_OK) (void)hipOk(hipStreamSynchronize(c->stream), "autotune");
+  cleanup();
+  // the grid, lists and marching-cubes buffers hold trial data now
+  c->gridValid = false;
+  c->stepDone = false;
+  c->mcSampled = c->mcCounted = c->mcEmitted = false;
+  best.trials = trials;
+  if (out) *out = best;
+  return rc;
+}
+
+int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
+  if (!c) return LF_ERR_ARG;
+  if (nsteps == 0 || (nsteps & 1u))
+    return fail(c, LF_ERR_ARG, "lf_graph_capture: nsteps must be even and > 0");
+  if (!c->stepDone || c->n == 0)
+    return fail(c, LF_ERR_STATE, "lf_graph_capture: run one eager step with these sizes first");
+  LF_HIP(c, hipSetDevice(c->device));
+  lf::dropGraph(c);
+  int rc = lf::prepareScenes(c);  // uploads now, not inside the capture
+  if (rc) return rc;
+  LF_HIP(c, hipStreamSynchronize(c->stream));
+  const bool prof = c->prof;
+  c->prof = false;
+  c->capturing = true;
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    c->capturing = false;
+    c->prof = prof;
+    return lf::hipFail(c, e, "hipStreamBeginCapture");
+  }
+  for (uint32_t s = 0; s < nsteps && rc == LF_OK; ++s) {
+    rc = lf::pbfStepOnce(c);
+    if (rc == LF_OK && m) rc = lf::mcSample(c, *m);
+    if (rc == LF_OK && m) rc = lf::mcClassifyCount(c, *m);
+    if (rc == LF_OK && m) rc = lf::mcEmit(c, *m);
+  }
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(c->stream, &g);
+  c->capturing = false;
+  c->prof = prof;
+  if (rc == LF_OK && e != hipSuccess) rc = lf::hipFail(c, e, "hipStreamEndCapture");
+  if (rc == LF_OK) {
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      c->graph = nullptr;
+      rc = lf::hipFail(c, e, "hipGraphInstantiate");
+    }
+  }
+  if (g) (void)hipGraphDestroy(g);
+  if (rc == LF_OK) c->graphSteps = int(nsteps);
+  return rc;
+}
+
+int lf_graph_launch(lf_ctx* c, uint32_t times) {
+  if (!c) return LF_ERR_ARG;
+  if (!c->graph) return fail(c, LF_ERR_STATE, "lf_graph_launch: no captured graph");
+  LF_HIP(c, hipSetDevice(c->device));
+  for (uint32_t t = 0; t < times; ++t) LF_HIP(c, hipGraphLaunch(c->graph, c->stream));
+  return LF_OK;
 }
 
 /* Keep marching-cubes case indices for lf_read(LF_ARR_CASES) (parity tests). */

@@ -13,42 +13,55 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// K1: p* and its Morton cell key per particle (state order).
-__global__ __launch_bounds__(kThreads) void k_predict_key(
-    uint32_t n, PbfConsts c, const float4* __restrict__ X, const float4* __restrict__ V,
-    const uint8_t* __restrict__ T, const lf_well* __restrict__ wells, int nWells,
-    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  float px, py, pz;
-  predictPstar(c, X[i], V[i], T[i], wells, nWells, px, py, pz);
-  keys[i] = cellKey(c, px, py, pz);
-  vals[i] = i;
+// K1: p* and its sort key per particle (state order). The sort key is the
+// scene's table base plus its Morton cell key (clamped to G in a batch of
+// several scenes, so scenes never overlap; a batch of one keeps the exact key).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_predict_key(Work W, const float4* __restrict__ X,
+                                                    const float4* __restrict__ V,
+                                                    const uint8_t* __restrict__ T,
+                                                    const lf_well* __restrict__ wells, int nWells,
+                                                    uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
+  forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    const uint32_t i = ch.begin + threadIdx.x;
+    if (i >= ch.end) return;
+    float px, py, pz;
+    predictPstar(c, X[i], V[i], T[i], wells, nWells, px, py, pz);
+    const uint32_t key = cellKey(c, px, py, pz);
+    keys[i] = c.keyBase + (c.clampKeys ? min(key, c.G) : key);
+    vals[i] = i;
+  });
 }
 
 // K3: gather the state into sorted order; p* is recomputed from the gathered
 // x, v (same arithmetic as K1, so identical bits) instead of being stored by
-// K1 and gathered. v stays the pre-gravity velocity: the reference hands the
-// device the velocity copied before the gravity update (oclsph.hpp:300,831).
-__global__ __launch_bounds__(kThreads) void k_reorder(
-    uint32_t n, PbfConsts c, const uint32_t* __restrict__ perm, const float4* __restrict__ Xin,
+// K1 and gathered, and with it the scene-local cell key the pair passes use.
+// v stays the pre-gravity velocity: the reference hands the device the
+// velocity copied before the gravity update (oclsph.hpp:300,831).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_reorder(
+    Work W, uint32_t n, const uint32_t* __restrict__ perm, const float4* __restrict__ Xin,
     const float4* __restrict__ Vin, const uint8_t* __restrict__ Tin,
     const uint32_t* __restrict__ IDin, const lf_well* __restrict__ wells, int nWells,
     float4* __restrict__ Xout, float4* __restrict__ Vout, uint8_t* __restrict__ Tout,
-    uint32_t* __restrict__ IDout, float4* __restrict__ P) {
-  const uint32_t s = blockIdx.x * kThreads + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t i = min(perm[s], n - 1u);  // perm is a permutation; the clamp only bounds reads
-  const float4 x = Xin[i];
-  const float4 v = Vin[i];
-  const uint8_t t = Tin[i];
-  float px, py, pz;
-  predictPstar(c, x, v, t, wells, nWells, px, py, pz);
-  Xout[s] = x;
-  Vout[s] = v;
-  Tout[s] = t;
-  IDout[s] = IDin[i];
-  P[s] = make_float4(px, py, pz, x.w);
+    uint32_t* __restrict__ IDout, float4* __restrict__ P, uint32_t* __restrict__ localKey) {
+  forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    const uint32_t s = ch.begin + threadIdx.x;
+    if (s >= ch.end) return;
+    const uint32_t i = min(perm[s], n - 1u);  // perm is a permutation; the clamp only bounds reads
+    const float4 x = Xin[i];
+    const float4 v = Vin[i];
+    const uint8_t t = Tin[i];
+    float px, py, pz;
+    predictPstar(c, x, v, t, wells, nWells, px, py, pz);
+    Xout[s] = x;
+    Vout[s] = v;
+    Tout[s] = t;
+    IDout[s] = IDin[i];
+    P[s] = make_float4(px, py, pz, x.w);
+    localKey[s] = cellKey(c, px, py, pz);
+  });
 }
 
 // Lower bound of key k in sorted keys[0, n).
@@ -67,7 +80,8 @@ __device__ inline uint32_t lowerBound(const uint32_t* __restrict__ keys, uint32_
 constexpr uint32_t kCells = 4096;            // cells per workgroup
 constexpr uint32_t kPerThread = kCells / kThreads;
 
-// K4: table[c] = first sorted index whose key >= c, for c in [0, G): the
+// K4: table[c] = first sorted index whose key >= c, for c in [0, G) (G: the
+// concatenated tables of all scenes): the
 // monotone scan of oclsph.hpp:758-765 as a per-tile lower bound. Each
 // workgroup owns kCells cells: segment starts of its key range are marked in
 // LDS, then a suffix-min fills empty cells with the next start.
@@ -157,8 +171,9 @@ int copyProbe(Ctx* c, size_t bytes, int reps, float* ms) {
 
 int gridBuild(Ctx* c) {
   const size_t n = c->n;
-  const PbfConsts& k = c->consts;
   int rc;
+  if ((rc = prepareScenes(c))) return rc;
+  const uint32_t G = c->tableTotal;
   const int nxt = 1 - c->cur;
   if ((rc = ensure(c, c->sortKey[0], n, "sort keys"))) return rc;
   if ((rc = ensure(c, c->sortKey[1], n, "sort keys"))) return rc;
@@ -170,34 +185,35 @@ int gridBuild(Ctx* c) {
   if ((rc = ensure(c, c->ID[nxt], n, "ids"))) return rc;
   if ((rc = ensure(c, c->P, n, "p*"))) return rc;
   if ((rc = ensure(c, c->PL, n, "p*+lambda"))) return rc;
+  if ((rc = ensure(c, c->localKey, n, "cell keys"))) return rc;
   if ((rc = ensure(c, c->nbr, nbrCapacity(n), "neighbour lists"))) return rc;
   if ((rc = ensure(c, c->nbrCount, n, "neighbour counts"))) return rc;
   if ((rc = ensure(c, c->P0, n, "list positions"))) return rc;
   if ((rc = ensure(c, c->maxDisp, 4, "displacement"))) return rc;
   if ((rc = ensure(c, c->rho, n, "density"))) return rc;
   if ((rc = ensure(c, c->diffused, n, "diffused"))) return rc;
-  if ((rc = ensure(c, c->table, size_t(k.G), "cell table"))) return rc;
-  if ((rc = ensure(c, c->cellDisp, size_t(k.G), "cell displacement"))) return rc;
+  if ((rc = ensure(c, c->table, size_t(G), "cell table"))) return rc;
+  if ((rc = ensure(c, c->cellDisp, size_t(G), "cell displacement"))) return rc;
   if (n == 0) return LF_OK;
-  const unsigned nb = blocksFor(n, kThreads);
-  rc = launch(c, "predict_key", k_predict_key, dim3(nb), dim3(kThreads), 0, uint32_t(n), k,
-              (const float4*)c->X[c->cur].p, (const float4*)c->V[c->cur].p,
-              (const uint8_t*)c->T[c->cur].p, (const lf_well*)c->wells.p, c->nWells,
-              c->sortKey[0].p, c->sortVal[0].p);
+  const Work W = workOf(c, -1);
+  rc = LF_DISPATCH_BS(c, "predict_key", k_predict_key, W, (const float4*)c->X[c->cur].p,
+                      (const float4*)c->V[c->cur].p, (const uint8_t*)c->T[c->cur].p,
+                      (const lf_well*)c->wells.p, c->nWells, c->sortKey[0].p, c->sortVal[0].p);
   if (rc) return rc;
   int idx = 0;
   if ((rc = radixSort(c, n, idx))) return rc;
-  rc = launch(c, "reorder", k_reorder, dim3(nb), dim3(kThreads), 0, uint32_t(n), k,
-              (const uint32_t*)c->sortVal[idx].p, (const float4*)c->X[c->cur].p,
-              (const float4*)c->V[c->cur].p, (const uint8_t*)c->T[c->cur].p,
-              (const uint32_t*)c->ID[c->cur].p, (const lf_well*)c->wells.p, c->nWells,
-              c->X[nxt].p, c->V[nxt].p, c->T[nxt].p, c->ID[nxt].p, c->P.p);
+  rc = LF_DISPATCH_BS(c, "reorder", k_reorder, W, uint32_t(n), (const uint32_t*)c->sortVal[idx].p,
+                      (const float4*)c->X[c->cur].p, (const float4*)c->V[c->cur].p,
+                      (const uint8_t*)c->T[c->cur].p, (const uint32_t*)c->ID[c->cur].p,
+                      (const lf_well*)c->wells.p, c->nWells, c->X[nxt].p, c->V[nxt].p,
+                      c->T[nxt].p, c->ID[nxt].p, c->P.p, c->localKey.p);
   if (rc) return rc;
-  rc = launch(c, "cell_table", k_cell_table, dim3(blocksFor(k.G, kCells)), dim3(kThreads), 0,
-              (const uint32_t*)c->sortKey[idx].p, uint32_t(n), k.G, c->table.p, c->cellDisp.p);
+  rc = launch(c, "cell_table", k_cell_table, dim3(blocksFor(G, kCells)), dim3(kThreads), 0,
+              (const uint32_t*)c->sortKey[idx].p, uint32_t(n), G, c->table.p, c->cellDisp.p);
   if (rc) return rc;
   c->keysIdx = idx;
-  c->gridG = k.G;
+  c->gridG = c->consts.G;
+  c->gridTotal = G;
   c->cur = nxt;
   c->gridValid = true;
   return LF_OK;

@@ -44,7 +44,60 @@ struct PbfConsts {
   uint32_t G;        // dense cell-table size Morton(extent) (oclsph.hpp:749)
   uint32_t extent[3];
   uint32_t flags;
+  // batches: this scene's slice of the concatenated cell tables. The sort key
+  // is keyBase + key, with key clamped to G when clampKeys (keys >= G belong
+  // to no cell), so scenes sort apart and each keeps the reference's order.
+  uint32_t keyBase;
+  uint32_t clampKeys;
 };
+
+// ---- work decomposition -------------------------------------------------------
+// Chunks of at most the workgroup size of consecutive particles of one scene
+// (sorted and unsorted order keep each scene's range), so the constants a
+// workgroup reads are uniform.
+struct Chunk {
+  uint32_t scene, begin, end, pad;
+};
+struct Work {
+  const PbfConsts* consts;  // per scene
+  const Chunk* chunks;
+  uint32_t nChunks;
+  uint32_t* queue;  // persistent launch: [0] next chunk, [1] blocks done; null: a workgroup per chunk
+};
+
+// Runs body(consts, chunk) for the chunks of this workgroup: one chunk per
+// workgroup, or, persistent, chunks taken from the shared counter until it
+// passes nChunks (every workgroup reaches that exit). The last workgroup to
+// leave resets the counters for the next launch, so no host-side clear is
+// needed (graph-safe).
+template <typename Body> __device__ inline void forEachChunk(const Work& W, Body&& body) {
+  if (!W.queue) {
+    const Chunk ch = W.chunks[blockIdx.x];
+    body(W.consts[ch.scene], ch);
+    return;
+  }
+  __shared__ uint32_t next;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) next = atomicAdd(&W.queue[0], 1u);
+    __syncthreads();
+    const uint32_t i = next;
+    if (i >= W.nChunks) break;
+    const Chunk ch = W.chunks[i];
+    body(W.consts[ch.scene], ch);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&W.queue[1], 1u) == gridDim.x - 1) {
+      W.queue[0] = 0u;
+      W.queue[1] = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// Counter pairs of the persistent launches (one per kernel kind).
+enum { kSlotList = 0, kSlotLambda = 1, kSlotDelta = 2, kSlotXsph = 3, kQueueSlots = 4 };
 
 int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string& err);
 
@@ -220,9 +273,33 @@ struct Ctx {
   bool stepDone = false;
   float skinFrac = 0.2f;   // neighbour-list skin / h
   bool listFresh = false;  // P still holds the predicted p* the lists were built from
-  PbfConsts consts{};
+  PbfConsts consts{};      // scene 0 (single-scene calls), or the scene of lf_mc_* calls
   lf_pbf_params params{};
-  uint32_t gridG = 0;  // table size of the last grid build
+  uint32_t gridG = 0;      // table size of the last grid build
+
+  // scenes (a single-scene context is a batch of one)
+  bool batch = false;                      // scenes and params fixed by lf_batch_upload
+  std::vector<lf_pbf_params> sceneParams;  // per scene
+  std::vector<PbfConsts> sceneConsts;      // per scene, folded
+  std::vector<uint32_t> sceneBegin;        // particle offsets, nScenes + 1 entries
+  uint32_t tableTotal = 0;                 // concatenated table size
+  DevBuf<PbfConsts> dConsts;
+  DevBuf<Chunk> chunks;
+  uint32_t nChunks = 0;
+  uint32_t chunkThreads = 256;             // workgroup size of the particle kernels
+  std::vector<PbfConsts> uploadedConsts;  // what dConsts holds
+  std::vector<uint32_t> uploadedBegin;     // scene offsets of the uploaded chunk list
+  uint32_t chunksBuiltFor = 0;             // chunkThreads the chunk list was built with
+  bool persistent = false;                 // work-queue launches for the pair kernels
+  uint32_t persistentBlocks = 2048;        // grid of a persistent launch
+  DevBuf<uint32_t> queues;                 // 2 counters per persistent launch slot
+  uint32_t gridTotal = 0;                  // tableTotal of the last grid build
+  float mcScale = 1.0f;                    // world scale of the marching-cubes output
+  DevBuf<uint32_t> localKey;               // scene-local Morton key per sorted particle
+  int mcScene = 0;
+  bool capturing = false;
+  hipGraphExec_t graph = nullptr;
+  int graphSteps = 0;
   int keysIdx = 0;  // which sortKey/sortVal holds the sorted result
 
   // marching cubes
@@ -251,6 +328,9 @@ struct Ctx {
 
 // Error helpers: record text, return status.
 int fail(Ctx* c, int code, const std::string& what);
+// Destroys the captured graph (its kernels hold addresses and constants that
+// are about to change).
+void dropGraph(Ctx* c);
 int hipFail(Ctx* c, hipError_t e, const char* where);
 #define LF_HIP(ctx, expr)                                      \
   do {                                                         \
@@ -260,12 +340,15 @@ int hipFail(Ctx* c, hipError_t e, const char* where);
 
 template <typename T> int ensure(Ctx* c, DevBuf<T>& b, size_t count, const char* what) {
   if (count <= b.cap) return LF_OK;
+  if (c->capturing)  // a graph would keep the old address
+    return fail(c, LF_ERR_STATE, std::string(what) + ": buffer growth during graph capture");
   if (b.p) {
     hipError_t e = hipFree(b.p);
     if (e != hipSuccess) return hipFail(c, e, what);
     b.p = nullptr;
     b.cap = 0;
   }
+  dropGraph(c);
   size_t want = count + count / 4 + 256;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b.p), want * sizeof(T));
   if (e != hipSuccess) return hipFail(c, e, what);
@@ -296,7 +379,34 @@ int launch(Ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 b
 
 inline unsigned blocksFor(size_t n, unsigned per) { return unsigned((n + per - 1) / per); }
 
+// Launch grid of a particle kernel: one workgroup per chunk, or a persistent
+// grid of at most persistentBlocks workgroups when the launch takes its chunks
+// from a work queue.
+inline dim3 gridOf(const Ctx* c, const Work& W) {
+  if (!W.queue) return dim3(W.nChunks);
+  return dim3(W.nChunks < c->persistentBlocks ? W.nChunks : c->persistentBlocks);
+}
+
+// Launches KERNEL<BS> with the context's workgroup size (128, 256 or 512
+// threads; chunks hold at most that many particles).
+#define LF_DISPATCH_BS(ctx, NAME, KERNEL, W, ...)                                          \
+  [&]() -> int {                                                                           \
+    switch ((ctx)->chunkThreads) {                                                         \
+      case 128:                                                                            \
+        return ::lf::launch(ctx, NAME, KERNEL<128>, ::lf::gridOf(ctx, W), dim3(128), 0, W, \
+                            __VA_ARGS__);                                                  \
+      case 512:                                                                            \
+        return ::lf::launch(ctx, NAME, KERNEL<512>, ::lf::gridOf(ctx, W), dim3(512), 0, W, \
+                            __VA_ARGS__);                                                  \
+      default:                                                                             \
+        return ::lf::launch(ctx, NAME, KERNEL<256>, ::lf::gridOf(ctx, W), dim3(256), 0, W, \
+                            __VA_ARGS__);                                                  \
+    }                                                                                      \
+  }()
+
 // ---- launchers (one .hip file each) ----------------------------------------
+int prepareScenes(Ctx* c);                    // lf_api.hip: consts + chunks to the device
+Work workOf(Ctx* c, int queueSlot);           // queueSlot < 0: one workgroup per chunk
 int gridBuild(Ctx* c);                        // lf_grid.hip
 int radixSort(Ctx* c, size_t n, int& outIdx);  // lf_sort.hip
 int pbfBuildLists(Ctx* c);                    // lf_pbf.hip

@@ -515,7 +515,11 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* _
 
 int mcSample(Ctx* c, const lf_mc_params& m) {
   if (!c->gridValid) return fail(c, LF_ERR_STATE, "lf_mc_sample: no grid (call lf_grid_build or lf_pbf_step)");
-  const PbfConsts& k = c->consts;
+  // the selected scene of a batch: its constants and its slice of the table
+  const PbfConsts k = (c->batch && size_t(c->mcScene) < c->sceneConsts.size())
+                          ? c->sceneConsts[c->mcScene]
+                          : c->consts;
+  const uint32_t* table = c->table.p + (c->batch ? k.keyBase : 0u);
   Lattice L{};
   if (m.field == LF_MC_FIELD_REFERENCE) {
     if (!(m.resolution > 0.0f)) return fail(c, LF_ERR_ARG, "lf_mc_sample: resolution must be > 0");
@@ -549,7 +553,7 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
     if ((rc = ensure(c, c->mcNormal, size_t(total), "mc normals"))) return rc;
     if ((rc = ensure(c, c->mcColour, size_t(total), "mc colours"))) return rc;
     rc = launch(c, "mc_reference_field", k_mc_reference, dim3(nb), dim3(kThreads), 0, total, L, k,
-                m.resolution, m.particle_size, m.particle_influence, (const uint32_t*)c->table.p,
+                m.resolution, m.particle_size, m.particle_influence, table,
                 (const uint8_t*)c->T[c->cur].p, (const float4*)c->X[c->cur].p,
                 (const float4*)c->V[c->cur].p, c->field.p, c->mcNormal.p, c->mcColour.p);
     c->mcAttr = 1;
@@ -559,10 +563,11 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
     const uint64_t tiles = uint64_t(tX) * tY * tZ;
     if (tiles >= (uint64_t(1) << 31)) return fail(c, LF_ERR_ARG, "lf_mc_sample: lattice too large");
     rc = launch(c, "mc_density_field", k_mc_density, dim3(unsigned(tiles)), dim3(kThreads), 0, L,
-                k, (const uint32_t*)c->table.p, (const float4*)c->P.p, c->field.p, tY, tZ);
+                k, table, (const float4*)c->P.p, c->field.p, tY, tZ);
     c->mcAttr = 0;
   }
   if (rc) return rc;
+  c->mcScale = k.scale;
   c->mcSampled = true;
   c->mcCounted = c->mcEmitted = false;
   return LF_OK;
@@ -617,7 +622,7 @@ int mcEmit(Ctx* c, const lf_mc_params& m) {
     A.origin[i] = c->mcOrigin[i];
     A.step[i] = c->mcStep[i];
   }
-  A.scale = c->consts.scale;
+  A.scale = c->mcScale;
   A.cap = c->triCap;
   A.attributes = attr;
   if (nbl > 0) {

@@ -23,33 +23,34 @@
 // Buffers (sorted order): P = (p*, mass), PL = (p*, lambda), X = (world
 // position, mass), V = (velocity, colour bits), T = type, P0 = clamped p* at
 // the list build.
+#include <algorithm>
+
 #include "lf_internal.hpp"
 
 namespace lf {
 namespace {
 
-constexpr int kThreads = 256;
 constexpr uint32_t kQueueCap = 24;  // LDS queue entries per lane
 constexpr uint32_t kDrainAt = 8;    // the wave drains once a lane holds this many
 
 // Visits the candidates of a particle in the reference order (27 cells of
 // `key`) and passes those with accept(b) to use(b), in visit order, through a
-// per-lane LDS queue (q[i * kThreads]) that the whole wave drains together once
+// per-lane LDS queue (q[i * BS]) that the whole wave drains together once
 // any lane holds kDrainAt entries: the stores of use() then issue as a few
 // wide instructions instead of one per candidate under divergence.
-template <typename Accept, typename Use>
+template <int BS, typename Accept, typename Use>
 __device__ inline void visitCompacted(uint32_t key, const uint32_t* __restrict__ table, uint32_t G,
                                       uint32_t* q, Accept&& accept, Use&& use) {
   uint32_t qn = 0;
   auto drain = [&]() {
-    for (uint32_t i = 0; i < qn; ++i) use(q[i * kThreads]);
+    for (uint32_t i = 0; i < qn; ++i) use(q[i * BS]);
     qn = 0;
   };
   forEachNeighbourCell(key, table, G, [&](uint32_t s, uint32_t e) {
     for (uint32_t b = s; b < e; ++b) {
       if (accept(b)) {
         if (qn == kQueueCap) drain();
-        q[qn * kThreads] = b;
+        q[qn * BS] = b;
         ++qn;
       }
     }
@@ -105,41 +106,44 @@ __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint3
 #pragma unroll
       for (int ix = 0; ix < 3; ++ix) {
         const uint32_t code = xs[ix] | ys[iy] | zs[iz];
-        if (code + 1u < c.G) m = max(m, L.cellDisp[code]);
+        if (code + 1u < c.G) m = max(m, L.cellDisp[c.keyBase + code]);
       }
   return da + __uint_as_float(m) < c.skinSafe;
 }
 
 // Neighbour lists of the predicted p* (see the file comment).
-__global__ __launch_bounds__(kThreads) void k_build_list(uint32_t n, PbfConsts c,
-                                                         const uint32_t* __restrict__ keys,
-                                                         const uint32_t* __restrict__ table,
-                                                         const uint8_t* __restrict__ T,
-                                                         const float4* __restrict__ P, Lists L) {
-  __shared__ uint32_t queue[kQueueCap * kThreads];
-  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
-  if (a < 4) L.maxDisp[a] = 0u;
-  if (a >= n) return;
-  const float4 q = P[a];
-  const float4 pa = make_float4(clampAxis(c, 0, q.x), clampAxis(c, 1, q.y), clampAxis(c, 2, q.z), q.w);
-  L.P0[a] = pa;
-  if (T[a] == LF_OBSTACLE) {
-    L.count[a] = 0;
-    return;
-  }
-  uint32_t cnt = 0;
-  visitCompacted(
-      keys[a], table, c.G, queue + threadIdx.x,
-      [&](uint32_t b) {
-        const float4 pb = P[b];
-        return dist2(pa.x - clampAxis(c, 0, pb.x), pa.y - clampAxis(c, 1, pb.y),
-                     pa.z - clampAxis(c, 2, pb.z)) <= c.listR2;
-      },
-      [&](uint32_t b) {
-        if (cnt < kMaxNbr) L.nbr[nbrSlot(a, cnt)] = b;
-        ++cnt;
-      });
-  L.count[a] = cnt;
+template <int BS>
+__global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __restrict__ keys,
+                                                   const uint32_t* __restrict__ table,
+                                                   const uint8_t* __restrict__ T,
+                                                   const float4* __restrict__ P, Lists L) {
+  __shared__ uint32_t queue[kQueueCap * BS];
+  if (blockIdx.x == 0 && threadIdx.x < 4) L.maxDisp[threadIdx.x] = 0u;
+  forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    const uint32_t a = ch.begin + threadIdx.x;
+    if (a >= ch.end) return;
+    const float4 q = P[a];
+    const float4 pa =
+        make_float4(clampAxis(c, 0, q.x), clampAxis(c, 1, q.y), clampAxis(c, 2, q.z), q.w);
+    L.P0[a] = pa;
+    if (T[a] == LF_OBSTACLE) {
+      L.count[a] = 0;
+      return;
+    }
+    uint32_t cnt = 0;
+    visitCompacted<BS>(
+        keys[a], table + c.keyBase, c.G, queue + threadIdx.x,
+        [&](uint32_t b) {
+          const float4 pb = P[b];
+          return dist2(pa.x - clampAxis(c, 0, pb.x), pa.y - clampAxis(c, 1, pb.y),
+                       pa.z - clampAxis(c, 2, pb.z)) <= c.listR2;
+        },
+        [&](uint32_t b) {
+          if (cnt < kMaxNbr) L.nbr[nbrSlot(a, cnt)] = b;
+          ++cnt;
+        });
+    L.count[a] = cnt;
+  });
 }
 
 // Accumulators of the lambda pass; KAHAN adds Neumaier compensation terms.
@@ -218,39 +222,39 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
 // sph_lambda (oclsph_kernel.h:305-332). Rejected candidates contribute
 // exactly zero in the reference, so walking only the accepted ones gives the
 // same sums in the same order.
-template <bool KAHAN>
-__global__ __launch_bounds__(kThreads) void k_lambda(uint32_t n, PbfConsts c,
-                                                     const uint32_t* __restrict__ keys,
-                                                     const uint32_t* __restrict__ table,
-                                                     const uint8_t* __restrict__ T,
-                                                     const float4* __restrict__ P,
-                                                     float4* __restrict__ PL,
-                                                     float* __restrict__ rhoOut, Lists L,
-                                                     int fresh) {
-  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
-  if (a >= n) return;
-  const float4 pa = P[a];
-  if (T[a] == LF_OBSTACLE) {
-    PL[a] = make_float4(pa.x, pa.y, pa.z, 0.0f);
-    rhoOut[a] = 0.0f;
-    return;
-  }
-  LambdaSums A;
-  forEachAccepted(
-      c, L, 1, fresh != 0, a, pa, keys[a], table, [&](uint32_t b) { return P[b]; },
-      [&](uint32_t, float4 pb, float d2) {
-        lambdaTerm<KAHAN>(c, pa.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, A);
-      });
-  if (KAHAN) {
-    A.gx += A.cgx;
-    A.gy += A.cgy;
-    A.gz += A.cgz;
-    A.rho += A.crho;
-  }
-  const float norm2 = dist2(A.gx, A.gy, A.gz);
-  const float C1 = A.rho / c.rho0 - 1.0f;
-  PL[a] = make_float4(pa.x, pa.y, pa.z, -C1 / (norm2 + c.cfm));
-  rhoOut[a] = A.rho;
+template <int BS, bool KAHAN>
+__global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restrict__ keys,
+                                               const uint32_t* __restrict__ table,
+                                               const uint8_t* __restrict__ T,
+                                               const float4* __restrict__ P,
+                                               float4* __restrict__ PL, float* __restrict__ rhoOut,
+                                               Lists L, int fresh) {
+  forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    const uint32_t a = ch.begin + threadIdx.x;
+    if (a >= ch.end) return;
+    const float4 pa = P[a];
+    if (T[a] == LF_OBSTACLE) {
+      PL[a] = make_float4(pa.x, pa.y, pa.z, 0.0f);
+      rhoOut[a] = 0.0f;
+      return;
+    }
+    LambdaSums A;
+    forEachAccepted(
+        c, L, 1, fresh != 0, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return P[b]; },
+        [&](uint32_t, float4 pb, float d2) {
+          lambdaTerm<KAHAN>(c, pa.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, A);
+        });
+    if (KAHAN) {
+      A.gx += A.cgx;
+      A.gy += A.cgy;
+      A.gz += A.cgz;
+      A.rho += A.crho;
+    }
+    const float norm2 = dist2(A.gx, A.gy, A.gz);
+    const float C1 = A.rho / c.rho0 - 1.0f;
+    PL[a] = make_float4(pa.x, pa.y, pa.z, -C1 / (norm2 + c.cfm));
+    rhoOut[a] = A.rho;
+  });
 }
 
 // One accepted pair of sph_delta (oclsph_kernel.h:355-358).
@@ -268,91 +272,97 @@ __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float d
 
 // sph_delta (oclsph_kernel.h:335-384) without colliders; Jacobi: reads PL,
 // writes Pout. deltaP is not stored (nothing reads it after the pass). Also
-// raises D to the new displacements (one atomic per wave).
-__global__ __launch_bounds__(kThreads) void k_delta(uint32_t n, PbfConsts c,
-                                                    const uint32_t* __restrict__ keys,
-                                                    const uint32_t* __restrict__ table,
-                                                    const uint8_t* __restrict__ T,
-                                                    const float4* __restrict__ PL,
-                                                    const float4* __restrict__ X, Lists L,
-                                                    int fresh, float4* __restrict__ Pout) {
-  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
-  float disp = 0.0f;
-  if (a < n) {
-    const float4 pa = PL[a];
-    const float mass = X[a].w;
-    if (T[a] == LF_OBSTACLE) {
-      Pout[a] = make_float4(pa.x, pa.y, pa.z, mass);
-    } else {
-      const float la = pa.w;
-      float ax = 0.0f, ay = 0.0f, az = 0.0f;
-      forEachAccepted(
-          c, L, 2, fresh != 0, a, pa, keys[a], table, [&](uint32_t b) { return PL[b]; },
-          [&](uint32_t, float4 pb, float d2) {
-            deltaTerm(c, la, pb.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, ax, ay, az);
-          });
-      // pos = (p* + dp) * scale, clamped to the bounds (:364-371), p* = pos/scale
-      const float acc[3] = {ax, ay, az};
-      const float p[3] = {pa.x, pa.y, pa.z};
-      float o[3];
+// raises D and the per-cell displacement to the new positions' (one atomic
+// per wave, and per run of equal keys).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict__ keys,
+                                              const uint32_t* __restrict__ table,
+                                              const uint8_t* __restrict__ T,
+                                              const float4* __restrict__ PL,
+                                              const float4* __restrict__ X, Lists L, int fresh,
+                                              float4* __restrict__ Pout) {
+  forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    const uint32_t a = ch.begin + threadIdx.x;
+    const bool valid = a < ch.end;
+    float disp = 0.0f;
+    if (valid) {
+      const float4 pa = PL[a];
+      const float mass = X[a].w;
+      if (T[a] == LF_OBSTACLE) {
+        Pout[a] = make_float4(pa.x, pa.y, pa.z, mass);
+      } else {
+        const float la = pa.w;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        forEachAccepted(
+            c, L, 2, fresh != 0, a, pa, keys[a], table + c.keyBase,
+            [&](uint32_t b) { return PL[b]; },
+            [&](uint32_t, float4 pb, float d2) {
+              deltaTerm(c, la, pb.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, ax, ay, az);
+            });
+        // pos = (p* + dp) * scale, clamped to the bounds (:364-371), p* = pos/scale
+        const float acc[3] = {ax, ay, az};
+        const float p[3] = {pa.x, pa.y, pa.z};
+        float o[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float pos = (p[k] + acc[k]) * c.scale;
-        pos = (c.minB[k] < pos) ? pos : c.minB[k];
-        pos = (pos < c.maxB[k]) ? pos : c.maxB[k];
-        o[k] = pos / c.scale;
+        for (int k = 0; k < 3; ++k) {
+          float pos = (p[k] + acc[k]) * c.scale;
+          pos = (c.minB[k] < pos) ? pos : c.minB[k];
+          pos = (pos < c.maxB[k]) ? pos : c.maxB[k];
+          o[k] = pos / c.scale;
+        }
+        Pout[a] = make_float4(o[0], o[1], o[2], mass);
+        const float4 p0 = L.P0[a];
+        disp = __builtin_sqrtf(dist2(o[0] - p0.x, o[1] - p0.y, o[2] - p0.z));
+        if (!(disp < 3.0e38f)) disp = 3.0e38f;  // NaN / inf: every list unusable
       }
-      Pout[a] = make_float4(o[0], o[1], o[2], mass);
-      const float4 p0 = L.P0[a];
-      disp = __builtin_sqrtf(dist2(o[0] - p0.x, o[1] - p0.y, o[2] - p0.z));
-      if (!(disp < 3.0e38f)) disp = 3.0e38f;  // NaN / inf: every list unusable
     }
-  }
-  // per-cell maximum: one atomic per run of equal keys within the wave
-  const uint32_t key = a < n ? keys[a] : 0xFFFFFFFFu;
-  const uint32_t prevKey = __shfl_up(key, 1, 64);
-  float runMax = disp;  // segmented max towards the first lane of each run
-  const bool head = __lane_id() == 0 || prevKey != key;
+    // per-cell maximum: one atomic per run of equal keys within the wave
+    const uint32_t key = valid ? keys[a] : 0xFFFFFFFFu;
+    const uint32_t prevKey = __shfl_up(key, 1, 64);
+    float runMax = disp;
+    const bool head = __lane_id() == 0 || prevKey != key;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const float other = __shfl_down(runMax, off, 64);
-    const uint32_t otherKey = __shfl_down(key, off, 64);
-    if (__lane_id() + off < 64 && otherKey == key) runMax = fmaxf(runMax, other);
-  }
-  if (a < n && head && runMax > 0.0f && key + 1u < c.G)
-    atomicMax(&L.cellDisp[key], __float_as_uint(runMax));
+    for (int off = 1; off < 64; off <<= 1) {
+      const float other = __shfl_down(runMax, off, 64);
+      const uint32_t otherKey = __shfl_down(key, off, 64);
+      if (__lane_id() + off < 64 && otherKey == key) runMax = fmaxf(runMax, other);
+    }
+    if (valid && head && runMax > 0.0f && key + 1u < c.G)
+      atomicMax(&L.cellDisp[c.keyBase + key], __float_as_uint(runMax));
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
-  if (__lane_id() == 0 && disp > 0.0f) atomicMax(L.maxDisp, __float_as_uint(disp));
+    for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
+    if (__lane_id() == 0 && disp > 0.0f) atomicMax(L.maxDisp, __float_as_uint(disp));
+  });
 }
 
 // sph_finalise (oclsph_kernel.h:404-416). Also sets P.w to the fluid mass (0
 // for obstacles) for the density field, and moves the diffused colour into V.w.
-__global__ __launch_bounds__(kThreads) void k_finalise(uint32_t n, PbfConsts c,
-                                                       const uint8_t* __restrict__ T,
-                                                       float4* __restrict__ P,
-                                                       float4* __restrict__ X,
-                                                       float4* __restrict__ V,
-                                                       const uint32_t* __restrict__ diffused,
-                                                       int useDiffused) {
-  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
-  if (a >= n) return;
-  float4 p = P[a];
-  if (T[a] == LF_OBSTACLE) {
-    p.w = 0.0f;
-    P[a] = p;
-    return;
-  }
-  const float4 x = X[a];
-  const float4 v = V[a];
-  const float dX = p.x - x.x / c.scale;
-  const float dY = p.y - x.y / c.scale;
-  const float dZ = p.z - x.z / c.scale;
-  X[a] = make_float4(p.x * c.scale, p.y * c.scale, p.z * c.scale, x.w);
-  const float col = useDiffused ? __uint_as_float(diffused[a]) : v.w;
-  V[a] = make_float4(__builtin_fmaf(dX, c.invDt, v.x) * c.vd,
-                     __builtin_fmaf(dY, c.invDt, v.y) * c.vd,
-                     __builtin_fmaf(dZ, c.invDt, v.z) * c.vd, col);
+template <int BS>
+__global__ __launch_bounds__(BS) void k_finalise(Work W, const uint8_t* __restrict__ T,
+                                                 float4* __restrict__ P, float4* __restrict__ X,
+                                                 float4* __restrict__ V,
+                                                 const uint32_t* __restrict__ diffused,
+                                                 int useDiffused) {
+  forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    const uint32_t a = ch.begin + threadIdx.x;
+    if (a >= ch.end) return;
+    float4 p = P[a];
+    if (T[a] == LF_OBSTACLE) {
+      p.w = 0.0f;
+      P[a] = p;
+      return;
+    }
+    const float4 x = X[a];
+    const float4 v = V[a];
+    const float dX = p.x - x.x / c.scale;
+    const float dY = p.y - x.y / c.scale;
+    const float dZ = p.z - x.z / c.scale;
+    X[a] = make_float4(p.x * c.scale, p.y * c.scale, p.z * c.scale, x.w);
+    const float col = useDiffused ? __uint_as_float(diffused[a]) : v.w;
+    V[a] = make_float4(__builtin_fmaf(dX, c.invDt, v.x) * c.vd,
+                       __builtin_fmaf(dY, c.invDt, v.y) * c.vd,
+                       __builtin_fmaf(dZ, c.invDt, v.z) * c.vd, col);
+  });
 }
 
 __device__ inline void unpackARGB(uint32_t c, float out[4]) {
@@ -370,76 +380,81 @@ __device__ inline uint32_t toU8(float v) {
 
 // sph_diffuse (oclsph_kernel.h:281-303): mean colour of the non-obstacle
 // candidates, mix(c_a, mean * 1.33, dt/750), clamp [8, 255], truncate.
-__global__ __launch_bounds__(kThreads) void k_diffuse(uint32_t n, PbfConsts c,
-                                                      const uint32_t* __restrict__ keys,
-                                                      const uint32_t* __restrict__ table,
-                                                      const uint8_t* __restrict__ T,
-                                                      const float4* __restrict__ V,
-                                                      uint32_t* __restrict__ diffused) {
-  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
-  if (a >= n) return;
-  const uint32_t ca = __float_as_uint(V[a].w);
-  if (T[a] == LF_OBSTACLE) {
-    diffused[a] = ca;
-    return;
-  }
-  float mix[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  int N = 0;
-  forEachNeighbourCell(keys[a], table, c.G, [&](uint32_t s, uint32_t e) {
-    for (uint32_t b = s; b < e; ++b) {
-      if (T[b] == LF_OBSTACLE) continue;
-      float cb[4];
-      unpackARGB(__float_as_uint(V[b].w), cb);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) mix[k] += cb[k];
-      ++N;
+template <int BS>
+__global__ __launch_bounds__(BS) void k_diffuse(Work W, const uint32_t* __restrict__ keys,
+                                                const uint32_t* __restrict__ table,
+                                                const uint8_t* __restrict__ T,
+                                                const float4* __restrict__ V,
+                                                uint32_t* __restrict__ diffused) {
+  forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    const uint32_t a = ch.begin + threadIdx.x;
+    if (a >= ch.end) return;
+    const uint32_t ca = __float_as_uint(V[a].w);
+    if (T[a] == LF_OBSTACLE) {
+      diffused[a] = ca;
+      return;
     }
-  });
-  float cf[4];
-  unpackARGB(ca, cf);
-  const float t = c.dt / 750.0f;
-  uint32_t out = 0;
+    float mix[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int N = 0;
+    forEachNeighbourCell(keys[a], table + c.keyBase, c.G, [&](uint32_t s, uint32_t e) {
+      for (uint32_t b = s; b < e; ++b) {
+        if (T[b] == LF_OBSTACLE) continue;
+        float cb[4];
+        unpackARGB(__float_as_uint(V[b].w), cb);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const float y = (mix[k] / float(N)) * 1.33f;
-    float o = __builtin_fmaf(y - cf[k], t, cf[k]);
-    o = fminf(fmaxf(o, 8.0f), 255.0f);
-    out = (out << 8) | toU8(o);
-  }
-  diffused[a] = out;
+        for (int k = 0; k < 4; ++k) mix[k] += cb[k];
+        ++N;
+      }
+    });
+    float cf[4];
+    unpackARGB(ca, cf);
+    const float t = c.dt / 750.0f;
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float y = (mix[k] / float(N)) * 1.33f;
+      float o = __builtin_fmaf(y - cf[k], t, cf[k]);
+      o = fminf(fmaxf(o, 8.0f), 255.0f);
+      out = (out << 8) | toU8(o);
+    }
+    diffused[a] = out;
+  });
 }
 
 // XSPH viscosity (absent in the reference): v_a += (c/rho0) sum_b m_b (v_b -
 // v_a) W6(r) over fluid b within h, Jacobi (Vin -> Vout). P.w is the fluid
 // mass after finalise.
-__global__ __launch_bounds__(kThreads) void k_xsph(uint32_t n, PbfConsts c,
-                                                   const uint32_t* __restrict__ keys,
-                                                   const uint32_t* __restrict__ table,
-                                                   const uint8_t* __restrict__ T,
-                                                   const float4* __restrict__ P,
-                                                   const float4* __restrict__ Vin, Lists L,
-                                                   float4* __restrict__ Vout) {
-  const uint32_t a = blockIdx.x * kThreads + threadIdx.x;
-  if (a >= n) return;
-  const float4 va = Vin[a];
-  if (T[a] == LF_OBSTACLE) {
-    Vout[a] = va;
-    return;
-  }
-  const float4 pa = P[a];
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  forEachAccepted(
-      c, L, 3, false, a, pa, keys[a], table, [&](uint32_t b) { return P[b]; },
-      [&](uint32_t b, float4 pb, float d2) {
-        if (T[b] == LF_OBSTACLE) return;
-        const float mw = pb.w * poly6W(c, __builtin_sqrtf(d2));
-        const float4 vb = Vin[b];
-        ax = __builtin_fmaf(mw, vb.x - va.x, ax);
-        ay = __builtin_fmaf(mw, vb.y - va.y, ay);
-        az = __builtin_fmaf(mw, vb.z - va.z, az);
-      });
-  Vout[a] = make_float4(__builtin_fmaf(c.xsphRho, ax, va.x), __builtin_fmaf(c.xsphRho, ay, va.y),
-                        __builtin_fmaf(c.xsphRho, az, va.z), va.w);
+template <int BS>
+__global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict__ keys,
+                                             const uint32_t* __restrict__ table,
+                                             const uint8_t* __restrict__ T,
+                                             const float4* __restrict__ P,
+                                             const float4* __restrict__ Vin, Lists L,
+                                             float4* __restrict__ Vout) {
+  forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    const uint32_t a = ch.begin + threadIdx.x;
+    if (a >= ch.end) return;
+    const float4 va = Vin[a];
+    if (T[a] == LF_OBSTACLE || c.xsph == 0.0f) {
+      Vout[a] = va;
+      return;
+    }
+    const float4 pa = P[a];
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    forEachAccepted(
+        c, L, 3, false, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return P[b]; },
+        [&](uint32_t b, float4 pb, float d2) {
+          if (T[b] == LF_OBSTACLE) return;
+          const float mw = pb.w * poly6W(c, __builtin_sqrtf(d2));
+          const float4 vb = Vin[b];
+          ax = __builtin_fmaf(mw, vb.x - va.x, ax);
+          ay = __builtin_fmaf(mw, vb.y - va.y, ay);
+          az = __builtin_fmaf(mw, vb.z - va.z, az);
+        });
+    Vout[a] = make_float4(__builtin_fmaf(c.xsphRho, ax, va.x),
+                          __builtin_fmaf(c.xsphRho, ay, va.y),
+                          __builtin_fmaf(c.xsphRho, az, va.z), va.w);
+  });
 }
 
 Lists listsOf(Ctx* c) {
@@ -450,57 +465,63 @@ Lists listsOf(Ctx* c) {
 
 int pbfBuildLists(Ctx* c) {
   if (c->n == 0) return LF_OK;
-  const unsigned nb = blocksFor(c->n, kThreads);
   c->listFresh = true;
-  return launch(c, "pbf_build_list", k_build_list, dim3(nb), dim3(kThreads), 0, uint32_t(c->n),
-                c->consts, (const uint32_t*)c->sortKey[c->keysIdx].p, (const uint32_t*)c->table.p,
-                (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, listsOf(c));
+  const Work W = workOf(c, kSlotList);
+  return LF_DISPATCH_BS(c, "pbf_build_list", k_build_list, W,
+                        (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
+                        (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, listsOf(c));
 }
 
 int pbfLambda(Ctx* c) {
   if (c->n == 0) return LF_OK;
-  const unsigned nb = blocksFor(c->n, kThreads);
-  const uint32_t* keys = c->sortKey[c->keysIdx].p;
-  if (c->consts.flags & LF_FLAG_KAHAN)
-    return launch(c, "pbf_lambda_kahan", k_lambda<true>, dim3(nb), dim3(kThreads), 0,
-                  uint32_t(c->n), c->consts, keys, (const uint32_t*)c->table.p,
-                  (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p,
-                  listsOf(c), int(c->listFresh));
-  return launch(c, "pbf_lambda", k_lambda<false>, dim3(nb), dim3(kThreads), 0, uint32_t(c->n),
-                c->consts, keys, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
-                (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), int(c->listFresh));
+  const Work W = workOf(c, kSlotLambda);
+  const int fresh = int(c->listFresh);
+  if (c->consts.flags & LF_FLAG_KAHAN) {
+    switch (c->chunkThreads) {
+      case 128: return launch(c, "pbf_lambda_kahan", k_lambda<128, true>, gridOf(c, W), dim3(128), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
+      case 512: return launch(c, "pbf_lambda_kahan", k_lambda<512, true>, gridOf(c, W), dim3(512), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
+      default: return launch(c, "pbf_lambda_kahan", k_lambda<256, true>, gridOf(c, W), dim3(256), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
+    }
+  }
+  switch (c->chunkThreads) {
+    case 128: return launch(c, "pbf_lambda", k_lambda<128, false>, gridOf(c, W), dim3(128), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
+    case 512: return launch(c, "pbf_lambda", k_lambda<512, false>, gridOf(c, W), dim3(512), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
+    default: return launch(c, "pbf_lambda", k_lambda<256, false>, gridOf(c, W), dim3(256), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
+  }
 }
 
 int pbfDelta(Ctx* c) {
   if (c->n == 0) return LF_OK;
-  const unsigned nb = blocksFor(c->n, kThreads);
-  const int rc = launch(c, "pbf_delta", k_delta, dim3(nb), dim3(kThreads), 0, uint32_t(c->n), c->consts,
-                (const uint32_t*)c->sortKey[c->keysIdx].p, (const uint32_t*)c->table.p,
-                (const uint8_t*)c->T[c->cur].p, (const float4*)c->PL.p,
-                (const float4*)c->X[c->cur].p, listsOf(c), int(c->listFresh), c->P.p);
+  const Work W = workOf(c, kSlotDelta);
+  const int rc = LF_DISPATCH_BS(c, "pbf_delta", k_delta, W, (const uint32_t*)c->localKey.p,
+                                (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
+                                (const float4*)c->PL.p, (const float4*)c->X[c->cur].p,
+                                listsOf(c), int(c->listFresh), c->P.p);
   c->listFresh = false;  // p* now differs from the predicted positions
   return rc;
 }
 
 int pbfFinalise(Ctx* c) {
   if (c->n == 0) return LF_OK;
-  const unsigned nb = blocksFor(c->n, kThreads);
-  return launch(c, "pbf_finalise", k_finalise, dim3(nb), dim3(kThreads), 0, uint32_t(c->n),
-                c->consts, (const uint8_t*)c->T[c->cur].p, c->P.p, c->X[c->cur].p,
-                c->V[c->cur].p, (const uint32_t*)c->diffused.p,
-                int((c->consts.flags & LF_FLAG_DIFFUSE) != 0));
+  const Work W = workOf(c, -1);
+  return LF_DISPATCH_BS(c, "pbf_finalise", k_finalise, W, (const uint8_t*)c->T[c->cur].p, c->P.p,
+                        c->X[c->cur].p, c->V[c->cur].p, (const uint32_t*)c->diffused.p,
+                        int((c->consts.flags & LF_FLAG_DIFFUSE) != 0));
 }
 
 int pbfXsph(Ctx* c) {
-  if (c->n == 0 || c->consts.xsph == 0.0f) return LF_OK;
-  const unsigned nb = blocksFor(c->n, kThreads);
+  if (c->n == 0) return LF_OK;
+  bool any = false;
+  for (const PbfConsts& k : c->sceneConsts) any = any || k.xsph != 0.0f;
+  if (!any) return LF_OK;
   const int nxt = 1 - c->cur;
   int rc = ensure(c, c->V[nxt], c->n, "velocities");
   if (rc) return rc;
-  rc = launch(c, "pbf_xsph", k_xsph, dim3(nb), dim3(kThreads), 0, uint32_t(c->n), c->consts,
-              (const uint32_t*)c->sortKey[c->keysIdx].p, (const uint32_t*)c->table.p,
-              (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p,
-              (const float4*)c->V[c->cur].p, listsOf(c), c->V[nxt].p);
+  const Work W = workOf(c, kSlotXsph);
+  rc = LF_DISPATCH_BS(c, "pbf_xsph", k_xsph, W, (const uint32_t*)c->localKey.p,
+                      (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
+                      (const float4*)c->P.p, (const float4*)c->V[c->cur].p, listsOf(c),
+                      c->V[nxt].p);
   if (rc) return rc;
   std::swap(c->V[0], c->V[1]);  // the XSPH output becomes the current V
   return LF_OK;
@@ -508,11 +529,10 @@ int pbfXsph(Ctx* c) {
 
 int pbfDiffuse(Ctx* c) {
   if (c->n == 0) return LF_OK;
-  const unsigned nb = blocksFor(c->n, kThreads);
-  return launch(c, "pbf_diffuse", k_diffuse, dim3(nb), dim3(kThreads), 0, uint32_t(c->n),
-                c->consts, (const uint32_t*)c->sortKey[c->keysIdx].p,
-                (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
-                (const float4*)c->V[c->cur].p, c->diffused.p);
+  const Work W = workOf(c, -1);
+  return LF_DISPATCH_BS(c, "pbf_diffuse", k_diffuse, W, (const uint32_t*)c->localKey.p,
+                        (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
+                        (const float4*)c->V[c->cur].p, c->diffused.p);
 }
 
 }  // namespace lf

@@ -152,3 +152,41 @@ def small_block(n_side=10, spacing=0.05, seed=7, box=(1.0, 1.0, 1.0), jitter=Tru
                  type=np.zeros(n, np.int32), colour=np.full(n, 0xFF2D1010, np.uint32), ids=ids,
                  min_bound=(0.0, 0.0, 0.0), max_bound=tuple(box), steps=2,
                  mc_samples=(32, 32, 32), seed=seed)
+
+
+def _block_dims(n, aspect):
+    """Integer lattice dims with product close to n and the given axis ratios."""
+    a = np.asarray(aspect, np.float64)
+    base = (n / np.prod(a)) ** (1.0 / 3.0)
+    dims = np.maximum(2, np.rint(base * a)).astype(int)
+    # adjust the longest axis so the count is as close to n as possible
+    k = int(np.argmax(dims))
+    rest = int(np.prod(dims)) // int(dims[k])
+    dims[k] = max(2, int(round(n / rest)))
+    return tuple(int(d) for d in dims)
+
+
+def batch_scenes(n_scenes=64, lo=2000, hi=200000, seed=3, iterations=4, spacing_range=(0.01, 0.05),
+                 box_factor=(2.0, 1.5, 1.25), mc_samples=(64, 64, 64)):
+    """Independent dam breaks of uneven size for batched stepping.
+
+    Particle counts are log-uniform in [lo, hi]; each scene is a jittered
+    lattice block of random aspect ratio (each axis x exp(U(-0.5, 0.5))) in the
+    corner of its own box (block extent x box_factor), with its own world
+    spacing (log-uniform in spacing_range) and so its own scale. Every scene
+    uses the solver-unit recipe of the module docstring (h = 2 spacings, about
+    33 neighbours within h)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), n_scenes))
+    out = []
+    for s in range(n_scenes):
+        aspect = np.exp(rng.uniform(-0.5, 0.5, 3))
+        dims = _block_dims(int(counts[s]), aspect)
+        sp = float(np.exp(rng.uniform(np.log(spacing_range[0]), np.log(spacing_range[1]))))
+        sp = float(np.float32(sp))
+        box = tuple(float(np.float32((d * sp) * f + 2 * sp)) for d, f in zip(dims, box_factor))
+        sc = _scene("batch_%d_%d" % (s, int(np.prod(dims))), [(dims, (sp / 2,) * 3)], sp, box,
+                    seed * 1000 + s, 10, mc_samples)
+        sc.iterations = iterations
+        out.append(sc)
+    return out
