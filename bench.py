@@ -76,6 +76,67 @@ def cpu_baseline(scene, mc, budget_s=20.0):
     return scene.n * steps / t, steps, threads
 
 
+def bench_batched(n_scenes, steps, warmup, seed=3, log=None):
+    """Aggregate particle-steps/s of a batch of uneven scenes, two ways:
+    plain = one context (own stream) per scene, stepped scene after scene
+    (one solver per scene, as libfluid runs); batched = all scenes in one
+    context, autotuned, whole steps replayed from a HIP graph."""
+    import libfluid_amd as lf
+    from libfluid_amd import scenes
+    sc = scenes.batch_scenes(n_scenes, seed=seed)
+    total = sum(s.n for s in sc)
+    log("batched: %d scenes, %d particles" % (len(sc), total))
+    # plain per-scene form
+    ctxs = []
+    for s in sc:
+        c = lf.Context(0)
+        c.upload(s.position, s.velocity, s.mass, s.type, s.colour)
+        ctxs.append((c, s.params()))
+    for _ in range(warmup):
+        for c, p in ctxs:
+            c.step(p, 1)
+    for c, _ in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c, p in ctxs:
+            c.step(p, 1)
+    for c, _ in ctxs:
+        c.sync()
+    plain_s = time.perf_counter() - t0
+    for c, _ in ctxs:
+        c.close()
+    log("batched: plain per-scene %.3f s" % plain_s)
+    # batched form
+    ctx = lf.Context(0)
+    ctx.batch_upload([s.params() for s in sc],
+                     [dict(position=s.position, velocity=s.velocity, mass=s.mass, type=s.type,
+                           colour=s.colour) for s in sc])
+    ctx.batch_step(1)
+    tune = ctx.autotune(3)
+    log("batched: autotune %s" % tune)
+    ctx.batch_step(1)
+    ctx.graph_capture(2)
+    reps = max(1, warmup // 2)
+    ctx.graph_launch(reps)
+    ctx.sync()
+    reps = max(1, steps // 2)
+    t0 = time.perf_counter()
+    ctx.graph_launch(reps)
+    ctx.sync()
+    batched_s = time.perf_counter() - t0
+    ctx.close()
+    log("batched: graph %.3f s" % batched_s)
+    plain = total * steps / plain_s
+    batched = total * 2 * reps / batched_s
+    return {
+        "workload": "%d scenes, %d particles (log-uniform %d..%d, seed %d), FP32, %d iterations"
+                    % (len(sc), total, min(s.n for s in sc), max(s.n for s in sc), seed,
+                       sc[0].iterations),
+        "plain_per_scene": plain, "batched_graph": batched, "speedup": batched / plain,
+        "unit": "particle-steps/s", "steps": steps, "autotune": tune}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +144,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--scene", default="dam_break_1m")
+    ap.add_argument("--batch-scenes", type=int, default=64,
+                    help="scenes of the batched comparison (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -170,6 +233,10 @@ def main():
         "kernels": kernels,
         "device_ms_per_step": device_ms / args.steps,
     }
+    if args.batch_scenes > 0:
+        out["batched"] = bench_batched(args.batch_scenes, max(4, args.steps // 5), args.warmup,
+                                       log=lambda m: print("bench: " + m, file=sys.stderr,
+                                                           flush=True))
     copy_bytes = 1 << 28
     copy_ms = ctx.copy_probe(copy_bytes, reps=20)
     out["copy_probe"] = {"bytes_moved": 2 * copy_bytes, "ms": copy_ms,

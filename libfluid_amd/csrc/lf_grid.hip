@@ -36,7 +36,8 @@ __global__ __launch_bounds__(BS) void k_predict_key(Work W, const float4* __rest
 
 // K3: gather the state into sorted order; p* is recomputed from the gathered
 // x, v (same arithmetic as K1, so identical bits) instead of being stored by
-// K1 and gathered, and with it the scene-local cell key the pair passes use.
+// K1 and gathered, and with it the scene-local cell key the pair passes use
+// and P0, p* clamped to the bounds, from which the neighbour lists are built.
 // v stays the pre-gravity velocity: the reference hands the device the
 // velocity copied before the gravity update (oclsph.hpp:300,831).
 template <int BS>
@@ -45,7 +46,8 @@ __global__ __launch_bounds__(BS) void k_reorder(
     const float4* __restrict__ Vin, const uint8_t* __restrict__ Tin,
     const uint32_t* __restrict__ IDin, const lf_well* __restrict__ wells, int nWells,
     float4* __restrict__ Xout, float4* __restrict__ Vout, uint8_t* __restrict__ Tout,
-    uint32_t* __restrict__ IDout, float4* __restrict__ P, uint32_t* __restrict__ localKey) {
+    uint32_t* __restrict__ IDout, float4* __restrict__ P, uint32_t* __restrict__ localKey,
+    float4* __restrict__ P0) {
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t s = ch.begin + threadIdx.x;
     if (s >= ch.end) return;
@@ -61,6 +63,8 @@ __global__ __launch_bounds__(BS) void k_reorder(
     IDout[s] = IDin[i];
     P[s] = make_float4(px, py, pz, x.w);
     localKey[s] = cellKey(c, px, py, pz);
+    P0[s] = make_float4(clampToBounds(c, 0, px), clampToBounds(c, 1, py), clampToBounds(c, 2, pz),
+                        x.w);
   });
 }
 
@@ -206,7 +210,7 @@ int gridBuild(Ctx* c) {
                       (const float4*)c->X[c->cur].p, (const float4*)c->V[c->cur].p,
                       (const uint8_t*)c->T[c->cur].p, (const uint32_t*)c->ID[c->cur].p,
                       (const lf_well*)c->wells.p, c->nWells, c->X[nxt].p, c->V[nxt].p,
-                      c->T[nxt].p, c->ID[nxt].p, c->P.p, c->localKey.p);
+                      c->T[nxt].p, c->ID[nxt].p, c->P.p, c->localKey.p, c->P0.p);
   if (rc) return rc;
   rc = launch(c, "cell_table", k_cell_table, dim3(blocksFor(G, kCells)), dim3(kThreads), 0,
               (const uint32_t*)c->sortKey[idx].p, uint32_t(n), G, c->table.p, c->cellDisp.p);

@@ -65,15 +65,27 @@ struct Work {
   uint32_t* queue;  // persistent launch: [0] next chunk, [1] blocks done; null: a workgroup per chunk
 };
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD
+// b % 8), each XCD with its own L2. This maps them so that XCD x works through
+// one contiguous eighth of the chunks: neighbouring particles, whose data the
+// pair passes gather, then share an L2. A bijection on [0, n).
+__device__ inline uint32_t xcdChunk(uint32_t b, uint32_t n) {
+  const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
 // Runs body(consts, chunk) for the chunks of this workgroup: one chunk per
 // workgroup, or, persistent, chunks taken from the shared counter until it
 // passes nChunks (every workgroup reaches that exit). The last workgroup to
 // leave resets the counters for the next launch, so no host-side clear is
 // needed (graph-safe).
 template <typename Body> __device__ inline void forEachChunk(const Work& W, Body&& body) {
+  // The constants are copied out of global memory once per chunk: a reference
+  // into global memory would be re-read after every store the body makes.
   if (!W.queue) {
-    const Chunk ch = W.chunks[blockIdx.x];
-    body(W.consts[ch.scene], ch);
+    const Chunk ch = W.chunks[xcdChunk(blockIdx.x, W.nChunks)];
+    const PbfConsts c = W.consts[ch.scene];
+    body(c, ch);
     return;
   }
   __shared__ uint32_t next;
@@ -84,7 +96,8 @@ template <typename Body> __device__ inline void forEachChunk(const Work& W, Body
     const uint32_t i = next;
     if (i >= W.nChunks) break;
     const Chunk ch = W.chunks[i];
-    body(W.consts[ch.scene], ch);
+    const PbfConsts c = W.consts[ch.scene];
+    body(c, ch);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -191,6 +204,14 @@ __device__ inline void forEachNeighbourCell(uint32_t key, const uint32_t* __rest
       }
     }
   }
+}
+
+// p* clamped to the bounds as the delta pass clamps it (oclsph_kernel.h:364-374).
+__device__ inline float clampToBounds(const PbfConsts& c, int k, float p) {
+  float pos = p * c.scale;
+  pos = (c.minB[k] < pos) ? pos : c.minB[k];
+  pos = (pos < c.maxB[k]) ? pos : c.maxB[k];
+  return pos / c.scale;
 }
 
 // ---- predict (advectAndCopy, oclsph.hpp:289-325) ---------------------------

@@ -69,14 +69,6 @@ struct Lists {
                            // of the particles of that cell
 };
 
-// p* clamped to the bounds as the delta pass clamps it (oclsph_kernel.h:364-374).
-__device__ inline float clampAxis(const PbfConsts& c, int k, float p) {
-  float pos = p * c.scale;
-  pos = (c.minB[k] < pos) ? pos : c.minB[k];
-  pos = (pos < c.maxB[k]) ? pos : c.maxB[k];
-  return pos / c.scale;
-}
-
 // Whether particle a may walk its list at position (x, y, z); `fresh` marks
 // the first lambda/delta pair after the build, where the lists are complete.
 __device__ inline bool listUsable(const PbfConsts& c, const Lists& L, uint32_t a, float x, float y,
@@ -111,32 +103,28 @@ __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint3
   return da + __uint_as_float(m) < c.skinSafe;
 }
 
-// Neighbour lists of the predicted p* (see the file comment).
+// Neighbour lists of the predicted p* (see the file comment). P0 (the
+// clamped p*) was written by the reorder kernel.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ table,
-                                                   const uint8_t* __restrict__ T,
-                                                   const float4* __restrict__ P, Lists L) {
+                                                   const uint8_t* __restrict__ T, Lists L) {
   __shared__ uint32_t queue[kQueueCap * BS];
   if (blockIdx.x == 0 && threadIdx.x < 4) L.maxDisp[threadIdx.x] = 0u;
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
     if (a >= ch.end) return;
-    const float4 q = P[a];
-    const float4 pa =
-        make_float4(clampAxis(c, 0, q.x), clampAxis(c, 1, q.y), clampAxis(c, 2, q.z), q.w);
-    L.P0[a] = pa;
     if (T[a] == LF_OBSTACLE) {
       L.count[a] = 0;
       return;
     }
+    const float4 pa = L.P0[a];
     uint32_t cnt = 0;
     visitCompacted<BS>(
         keys[a], table + c.keyBase, c.G, queue + threadIdx.x,
         [&](uint32_t b) {
-          const float4 pb = P[b];
-          return dist2(pa.x - clampAxis(c, 0, pb.x), pa.y - clampAxis(c, 1, pb.y),
-                       pa.z - clampAxis(c, 2, pb.z)) <= c.listR2;
+          const float4 pb = L.P0[b];
+          return dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z) <= c.listR2;
         },
         [&](uint32_t b) {
           if (cnt < kMaxNbr) L.nbr[nbrSlot(a, cnt)] = b;
@@ -281,6 +269,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
                                               const float4* __restrict__ PL,
                                               const float4* __restrict__ X, Lists L, int fresh,
                                               float4* __restrict__ Pout) {
+  __shared__ float waveMax[BS / 64];
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
     const bool valid = a < ch.end;
@@ -331,7 +320,19 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
       atomicMax(&L.cellDisp[c.keyBase + key], __float_as_uint(runMax));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
-    if (__lane_id() == 0 && disp > 0.0f) atomicMax(L.maxDisp, __float_as_uint(disp));
+    // D: the workgroup's maximum, then one atomic, and only when it raises D
+    // (D only grows during the pass, so a value read here is a lower bound)
+    if (__lane_id() == 0) waveMax[threadIdx.x >> 6] = disp;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = 0.0f;
+#pragma unroll
+      for (int w = 0; w < BS / 64; ++w) m = fmaxf(m, waveMax[w]);
+      const uint32_t bits = __float_as_uint(m);
+      if (m > 0.0f && __hip_atomic_load(L.maxDisp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bits)
+        atomicMax(L.maxDisp, bits);
+    }
+    __syncthreads();
   });
 }
 
@@ -422,8 +423,8 @@ __global__ __launch_bounds__(BS) void k_diffuse(Work W, const uint32_t* __restri
 }
 
 // XSPH viscosity (absent in the reference): v_a += (c/rho0) sum_b m_b (v_b -
-// v_a) W6(r) over fluid b within h, Jacobi (Vin -> Vout). P.w is the fluid
-// mass after finalise.
+// v_a) W6(r) over b within h, Jacobi (Vin -> Vout), with m_b = P.w, the fluid
+// mass after finalise (0 for obstacles).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict__ keys,
                                              const uint32_t* __restrict__ table,
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict_
     forEachAccepted(
         c, L, 3, false, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return P[b]; },
         [&](uint32_t b, float4 pb, float d2) {
-          if (T[b] == LF_OBSTACLE) return;
+          // P.w is the fluid mass, 0 for obstacles: they add zero-weight terms
           const float mw = pb.w * poly6W(c, __builtin_sqrtf(d2));
           const float4 vb = Vin[b];
           ax = __builtin_fmaf(mw, vb.x - va.x, ax);
@@ -469,7 +470,7 @@ int pbfBuildLists(Ctx* c) {
   const Work W = workOf(c, kSlotList);
   return LF_DISPATCH_BS(c, "pbf_build_list", k_build_list, W,
                         (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
-                        (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, listsOf(c));
+                        (const uint8_t*)c->T[c->cur].p, listsOf(c));
 }
 
 int pbfLambda(Ctx* c) {

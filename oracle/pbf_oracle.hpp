@@ -403,12 +403,12 @@ template <typename F> class Pbf {
       const F* va = &vs[3 * a];
       F acc[3] = {0, 0, 0};
       forEachCandidate(keys[a], [&](uint32_t b) {
-        if (ts[b] == LF_OBSTACLE) return;
         const F* pb = &ps[4 * b];
         const F dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
         const F d = std::sqrt(dist2(dx, dy, dz));
         if (!(d <= c.h)) return;
-        const F mw = ms[b] * c.W6(d);
+        // obstacles carry zero weight (their fluid mass is 0)
+        const F mw = (ts[b] == LF_OBSTACLE ? F(0) : ms[b]) * c.W6(d);
         for (int k = 0; k < 3; ++k) acc[k] = std::fma(mw, vs[3 * b + k] - va[k], acc[k]);
       });
       for (int k = 0; k < 3; ++k) out[3 * a + k] = std::fma(c.xsphRho, acc[k], va[k]);
