@@ -44,6 +44,11 @@ extern "C" {
                                   (sph_diffuse, oclsph_kernel.h:281-303) */
 #define LF_FLAG_KAHAN 0x2u     /* Kahan-compensated density and lambda-denominator
                                   sums in the lambda pass (mixed precision) */
+#define LF_FLAG_FAST_MATH 0x4u /* hardware sqrt and reciprocal (1 ulp) in the
+                                  lambda and delta pair terms instead of the
+                                  correctly rounded ones: results are no longer
+                                  bit-identical to the CPU restatement, only
+                                  close (mixed precision) */
 
 /*
  * Parameters of one PBF step. Replaces ClSphConfig (oclsph_type.h:6-13), the
@@ -66,6 +71,13 @@ typedef struct lf_pbf_params {
   float xsph_c;         /* XSPH viscosity c: v += (c/rho0) sum m_j (v_j - v) W6;
                            0 = off (the reference has none) */
   uint32_t flags;       /* LF_FLAG_* */
+  float density_tol;    /* > 0: density-error convergence check. After each
+                           lambda pass the mean |rho/rho0 - 1| over the
+                           scene's fluid particles is reduced in FP64 on the
+                           device; once it is below density_tol the scene's
+                           remaining lambda/delta passes of the step are
+                           skipped. 0 = always run `iterations` passes (the
+                           reference) */
 } lf_pbf_params;
 
 /* Fills p with the reference's constants for kernel radius h (dt 1/120,
@@ -222,6 +234,26 @@ int lf_batch_step(lf_ctx* ctx, uint32_t nsteps);
 int lf_batch_info(const lf_ctx* ctx, uint32_t* n_scenes, size_t* offsets);
 /* Scene of the following lf_mc_sample (params NULL) and lf_grid_info calls. */
 int lf_batch_select(lf_ctx* ctx, uint32_t scene);
+
+/* ---- FP64 diagnostics ------------------------------------------------------
+ * Conserved quantities and the density error of one scene, reduced on the
+ * device in FP64 (per-chunk partial sums, then a fixed-order sum per scene,
+ * so the figures are reproducible). No reference counterpart. */
+typedef struct lf_diag {
+  uint64_t fluid;             /* fluid particles */
+  double mass;                /* sum of fluid masses */
+  double momentum[3];         /* sum of m v over fluid particles */
+  double density_error_mean;  /* mean |rho/rho0 - 1| of the last lambda pass */
+  double density_error_max;   /* max  |rho/rho0 - 1| of the last lambda pass */
+  uint32_t passes;            /* lambda passes the scene ran in the last step */
+  uint32_t pad;
+  double pass_error[16];      /* with density_tol > 0: mean |rho/rho0 - 1| after
+                                 each of the first 16 lambda passes of the last
+                                 step (0 for passes not run) */
+} lf_diag;
+/* Diagnostics of scene `scene` (0 for a single-scene context) for the
+ * current state (synchronises). */
+int lf_diagnostics(lf_ctx* ctx, uint32_t scene, lf_diag* out);
 
 /* ---- execution configuration ----------------------------------------------
  * threads: workgroup size of the particle kernels (128, 256 or 512; each

@@ -27,14 +27,14 @@ from . import _abi
 from ._abi import (LF_ARR_CASES, LF_ARR_DENSITY, LF_ARR_DIFFUSED, LF_ARR_FIELD, LF_ARR_KEYS,
                    LF_ARR_LAMBDA, LF_ARR_MCCOLOUR, LF_ARR_MCNORMAL, LF_ARR_ORDER, LF_ARR_POSITION,
                    LF_ARR_PSTAR, LF_ARR_TABLE, LF_ARR_TYPE, LF_ARR_VELOCITY, LF_FLAG_DIFFUSE,
-                   LF_FLAG_KAHAN, LF_FLUID, LF_MC_FIELD_DENSITY, LF_MC_FIELD_REFERENCE,
+                   LF_FLAG_FAST_MATH, LF_FLAG_KAHAN, LF_FLUID, LF_MC_FIELD_DENSITY, LF_MC_FIELD_REFERENCE,
                    LF_OBSTACLE, McParams, PbfParams, particle_dtype, triangle_dtype)
 
 __all__ = [
     "Context", "SphSolver", "Config", "Source", "Drain", "Query", "QueryResult", "WellSpec",
     "PbfParams", "McParams", "LfError", "pbf_params", "mc_reference_params",
     "mc_density_params", "particle_dtype", "triangle_dtype", "LF_FLUID", "LF_OBSTACLE",
-    "LF_FLAG_DIFFUSE", "LF_FLAG_KAHAN",
+    "LF_FLAG_DIFFUSE", "LF_FLAG_KAHAN", "LF_FLAG_FAST_MATH",
 ]
 
 
@@ -48,7 +48,7 @@ class LfError(RuntimeError):
 
 def pbf_params(h, dt=1.0 / 120.0, scale=1.0, iterations=4, gravity=(0.0, -9.8, 0.0),
                min_bound=(0.0, 0.0, 0.0), max_bound=(1.0, 1.0, 1.0), xsph_c=0.0, flags=0,
-               rest_density=6378.0):
+               rest_density=6378.0, density_tol=0.0):
     """lf_pbf_params with the reference's kernel constants (oclsph_kernel.h:20-31)."""
     p = PbfParams()
     _abi.load().lf_pbf_params_default(ctypes.byref(p), ctypes.c_float(h))
@@ -61,6 +61,7 @@ def pbf_params(h, dt=1.0 / 120.0, scale=1.0, iterations=4, gravity=(0.0, -9.8, 0
     p.xsph_c = xsph_c
     p.flags = flags
     p.rest_density = rest_density
+    p.density_tol = density_tol
     return p
 
 
@@ -223,6 +224,15 @@ class Context:
 
     def batch_select(self, scene):
         self._check(self._lib.lf_batch_select(self._h, scene))
+
+    def diagnostics(self, scene=0):
+        """FP64 per-scene fluid count, mass, momentum and density error."""
+        d = _abi.Diag()
+        self._check(self._lib.lf_diagnostics(self._h, scene, ctypes.byref(d)))
+        return dict(fluid=d.fluid, mass=d.mass, momentum=tuple(d.momentum),
+                    density_error_mean=d.density_error_mean,
+                    density_error_max=d.density_error_max, passes=d.passes,
+                    pass_error=list(d.pass_error)[:max(0, d.passes)])
 
     def exec_config(self, threads=256, persistent=False, skin=0.2):
         self._check(self._lib.lf_exec_config(self._h, threads, 1 if persistent else 0, skin))

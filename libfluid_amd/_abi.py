@@ -23,6 +23,7 @@ LF_OBSTACLE = 1
 
 LF_FLAG_DIFFUSE = 0x1
 LF_FLAG_KAHAN = 0x2
+LF_FLAG_FAST_MATH = 0x4
 
 LF_MC_FIELD_REFERENCE = 0
 LF_MC_FIELD_DENSITY = 1
@@ -65,6 +66,7 @@ class PbfParams(ctypes.Structure):
         ("vd", ctypes.c_float),
         ("xsph_c", ctypes.c_float),
         ("flags", ctypes.c_uint32),
+        ("density_tol", ctypes.c_float),
     ]
 
 
@@ -103,6 +105,20 @@ class MeshTriangle(ctypes.Structure):
         ("v0", _f3), ("v1", _f3), ("v2", _f3),
         ("n0", _f3), ("n1", _f3), ("n2", _f3),
         ("c", _u3),
+    ]
+
+
+class Diag(ctypes.Structure):
+    """lf_diag: FP64 per-scene conserved quantities and density error."""
+    _fields_ = [
+        ("fluid", ctypes.c_uint64),
+        ("mass", ctypes.c_double),
+        ("momentum", ctypes.c_double * 3),
+        ("density_error_mean", ctypes.c_double),
+        ("density_error_max", ctypes.c_double),
+        ("passes", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("pass_error", ctypes.c_double * 16),
     ]
 
 
@@ -187,6 +203,7 @@ SIGNATURES = {
     "lf_autotune": (_I, [_P, _U32, ctypes.POINTER(TuneResult)]),
     "lf_graph_capture": (_I, [_P, _U32, _P]),
     "lf_graph_launch": (_I, [_P, _U32]),
+    "lf_diagnostics": (_I, [_P, _U32, ctypes.POINTER(Diag)]),
 }
 
 _lib = None

@@ -64,7 +64,9 @@ int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string
     c.skinSafe = float(skin * (1.0 - 1e-5));
   }
   c.flags = p.flags;
+  c.densityTol = p.density_tol > 0.0f ? p.density_tol : 0.0f;
   c.wdq = hostW6(c, p.corr_dq * c.h);
+  c.wdqRecip = 1.0f / c.wdq;
   // rmax2: the largest float x with sqrtf(x) <= h
   float x = c.hh;
   while (std::sqrt(x) > c.h) x = std::nextafter(x, 0.0f);
@@ -171,6 +173,8 @@ int prepareScenes(Ctx* c) {
   const bool sameChunks = begin == c->uploadedBegin && c->chunksBuiltFor == c->chunkThreads;
   c->sceneConsts = consts;
   c->tableTotal = uint32_t(total);
+  c->converge = false;
+  for (const PbfConsts& k : consts) c->converge = c->converge || k.densityTol > 0.0f;
   if (!c->batch) c->sceneBegin = begin;
   if (sameConsts && sameChunks && c->dConsts.p && c->chunks.p) return LF_OK;
   if (c->capturing)
@@ -185,10 +189,28 @@ int prepareScenes(Ctx* c) {
   std::vector<Chunk> chunks;
   if (!sameChunks) {
     const uint32_t B = c->chunkThreads;
-    for (size_t s = 0; s + 1 < begin.size(); ++s)
+    c->chunkOffHost.assign(begin.size(), 0u);
+    for (size_t s = 0; s + 1 < begin.size(); ++s) {
+      c->chunkOffHost[s] = uint32_t(chunks.size());
       for (uint32_t a = begin[s]; a < begin[s + 1]; a += B)
-        chunks.push_back(Chunk{uint32_t(s), a, std::min(a + B, begin[s + 1]), 0u});
+        chunks.push_back(
+            Chunk{uint32_t(s), a, std::min(a + B, begin[s + 1]), uint32_t(chunks.size())});
+    }
+    c->chunkOffHost.back() = uint32_t(chunks.size());
     if ((rc = ensure(c, c->chunks, std::max<size_t>(chunks.size(), 1), "chunks"))) return rc;
+    if ((rc = ensure(c, c->chunkOff, c->chunkOffHost.size(), "chunk offsets"))) return rc;
+    LF_HIP(c, hipMemcpyAsync(c->chunkOff.p, c->chunkOffHost.data(),
+                             c->chunkOffHost.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             c->stream));
+    const size_t S = begin.size() - 1;
+    if ((rc = ensure(c, c->chunkSums, std::max<size_t>(chunks.size(), 1) * kSumWidth, "chunk sums")))
+      return rc;
+    if ((rc = ensure(c, c->sceneStats, S * kStatSlots * kSumWidth, "scene statistics"))) return rc;
+    if ((rc = ensure(c, c->sceneDone, S, "scene flags"))) return rc;
+    if ((rc = ensure(c, c->scenePasses, S, "scene passes"))) return rc;
+    LF_HIP(c, hipMemsetAsync(c->sceneStats.p, 0, S * kStatSlots * kSumWidth * sizeof(double), c->stream));
+    LF_HIP(c, hipMemsetAsync(c->sceneDone.p, 0, S * sizeof(uint32_t), c->stream));
+    LF_HIP(c, hipMemsetAsync(c->scenePasses.p, 0, S * sizeof(uint32_t), c->stream));
     if (!chunks.empty())
       LF_HIP(c, hipMemcpyAsync(c->chunks.p, chunks.data(), chunks.size() * sizeof(Chunk),
                                hipMemcpyHostToDevice, c->stream));
@@ -205,8 +227,9 @@ int prepareScenes(Ctx* c) {
   return LF_OK;
 }
 
-Work workOf(Ctx* c, int queueSlot) {
+Work workOf(Ctx* c, int queueSlot, bool skipDone) {
   Work W{};
+  W.skip = (skipDone && c->converge) ? c->sceneDone.p : nullptr;
   W.consts = c->dConsts.p;
   W.chunks = c->chunks.p;
   W.nChunks = c->nChunks;
@@ -235,8 +258,9 @@ static int pbfStepOnce(Ctx* c) {
   if (c->consts.flags & LF_FLAG_DIFFUSE)
     if ((rc = pbfDiffuse(c))) return rc;
   for (uint32_t it = 0; it < c->params.iterations; ++it) {
-    if ((rc = pbfLambda(c))) return rc;
-    if ((rc = pbfDelta(c))) return rc;
+    if ((rc = pbfLambda(c, it > 0))) return rc;
+    if (c->converge && (rc = densityCheck(c, int(it)))) return rc;
+    if ((rc = pbfDelta(c, true))) return rc;
   }
   if ((rc = pbfFinalise(c))) return rc;
   if ((rc = pbfXsph(c))) return rc;
@@ -334,6 +358,11 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->chunks);
   lf::release(c->queues);
   lf::release(c->localKey);
+  lf::release(c->chunkOff);
+  lf::release(c->chunkSums);
+  lf::release(c->sceneStats);
+  lf::release(c->sceneDone);
+  lf::release(c->scenePasses);
   lf::dropGraph(c);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -471,8 +500,10 @@ int lf_grid_build(lf_ctx* c, const lf_pbf_params* p) {
   return fn(c);
 
 int lf_pbf_diffuse(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(lf::pbfDiffuse, "lf_pbf_diffuse") }
-int lf_pbf_lambda(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(lf::pbfLambda, "lf_pbf_lambda") }
-int lf_pbf_delta(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(lf::pbfDelta, "lf_pbf_delta") }
+static int lambdaStage(Ctx* c) { return lf::pbfLambda(c, false); }
+static int deltaStage(Ctx* c) { return lf::pbfDelta(c, false); }
+int lf_pbf_lambda(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(lambdaStage, "lf_pbf_lambda") }
+int lf_pbf_delta(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(deltaStage, "lf_pbf_delta") }
 int lf_pbf_finalise(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(lf::pbfFinalise, "lf_pbf_finalise") }
 int lf_pbf_xsph(lf_ctx* c, const lf_pbf_params* p) { LF_STAGE(lf::pbfXsph, "lf_pbf_xsph") }
 
@@ -1007,6 +1038,34 @@ int lf_graph_launch(lf_ctx* c, uint32_t times) {
   if (!c->graph) return fail(c, LF_ERR_STATE, "lf_graph_launch: no captured graph");
   LF_HIP(c, hipSetDevice(c->device));
   for (uint32_t t = 0; t < times; ++t) LF_HIP(c, hipGraphLaunch(c->graph, c->stream));
+  return LF_OK;
+}
+
+int lf_diagnostics(lf_ctx* c, uint32_t scene, lf_diag* out) {
+  if (!c || !out) return LF_ERR_ARG;
+  if (!c->gridValid) return fail(c, LF_ERR_STATE, "lf_diagnostics: no step or grid yet");
+  const uint32_t S = uint32_t(c->sceneConsts.size());
+  if (scene >= S) return fail(c, LF_ERR_ARG, "lf_diagnostics: no such scene");
+  LF_HIP(c, hipSetDevice(c->device));
+  int rc = lf::diagState(c);
+  if (rc) return rc;
+  std::vector<double> st(lf::kStatSlots * lf::kSumWidth);
+  uint32_t passes = 0;
+  LF_HIP(c, hipMemcpyAsync(st.data(), c->sceneStats.p + size_t(scene) * st.size(),
+                           st.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  LF_HIP(c, hipMemcpyAsync(&passes, c->scenePasses.p + scene, 4, hipMemcpyDeviceToHost, c->stream));
+  LF_HIP(c, hipStreamSynchronize(c->stream));
+  std::memset(out, 0, sizeof(*out));
+  const double* s = st.data() + size_t(lf::kPassSlots) * lf::kSumWidth;
+  out->fluid = uint64_t(s[0]);
+  out->mass = s[1];
+  out->momentum[0] = s[2];
+  out->momentum[1] = s[3];
+  out->momentum[2] = s[4];
+  out->density_error_mean = s[0] > 0.0 ? s[5] / s[0] : 0.0;
+  out->density_error_max = s[6];
+  out->passes = c->sceneConsts[scene].densityTol > 0.0f ? passes : c->params.iterations;
+  for (int i = 0; i < lf::kPassSlots; ++i) out->pass_error[i] = st[size_t(i) * lf::kSumWidth + 5];
   return LF_OK;
 }
 

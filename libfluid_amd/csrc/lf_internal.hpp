@@ -26,7 +26,13 @@
 //    (oclsph_kernel.h:50-71);
 //  * "r <= h" is tested on d2 against rmax2, the largest float whose
 //    correctly rounded square root is <= h, which is the same predicate
-//    without the square root.
+//    without the square root;
+//  * divisions by constants are multiplications by the reciprocal folded on
+//    the host (W6(r) / W6(dq) and / rho0 of sph_delta);
+//  * XSPH and the density field take W6 on d2 (t = h*h - d2, no sqrt).
+// The FAST variants of the pair kernels (LF_FLAG_FAST_MATH) replace the
+// correctly rounded sqrt and 1/r by the hardware approximations (1 ulp);
+// they are checked against the oracle with a tolerance instead.
 // ---------------------------------------------------------------------------
 
 namespace lf {
@@ -34,8 +40,8 @@ namespace lf {
 // Constants folded on the host from lf_pbf_params (FP32, the reference's
 // folding order of oclsph_kernel.h:20-51).
 struct PbfConsts {
-  float h, hh, rmax2, poly6, spiky, rhoRecip, rho0, cfm, corrK, wdq, invDt, dt, scale, vd,
-      xsph, xsphRho;  // xsphRho = xsph / rho0
+  float h, hh, rmax2, poly6, spiky, rhoRecip, rho0, cfm, corrK, wdq, wdqRecip, invDt, dt, scale,
+      vd, xsph, xsphRho;  // xsphRho = xsph / rho0
   float listR2;      // neighbour-list radius^2, (h + skin)^2 with rounding margin
   float skinSafe;    // skin less a rounding margin
   float minExt[3];   // minBound/scale - 2h (oclsph.hpp:359)
@@ -49,6 +55,7 @@ struct PbfConsts {
   // to no cell), so scenes sort apart and each keeps the reference's order.
   uint32_t keyBase;
   uint32_t clampKeys;
+  float densityTol;  // > 0: skip the remaining passes once the density error is below it
 };
 
 // ---- work decomposition -------------------------------------------------------
@@ -56,13 +63,14 @@ struct PbfConsts {
 // (sorted and unsorted order keep each scene's range), so the constants a
 // workgroup reads are uniform.
 struct Chunk {
-  uint32_t scene, begin, end, pad;
+  uint32_t scene, begin, end, index;  // index: position in the chunk list
 };
 struct Work {
   const PbfConsts* consts;  // per scene
   const Chunk* chunks;
   uint32_t nChunks;
   uint32_t* queue;  // persistent launch: [0] next chunk, [1] blocks done; null: a workgroup per chunk
+  const uint32_t* skip;  // per scene: non-zero skips the scene's chunks (converged); may be null
 };
 
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD
@@ -84,6 +92,7 @@ template <typename Body> __device__ inline void forEachChunk(const Work& W, Body
   // into global memory would be re-read after every store the body makes.
   if (!W.queue) {
     const Chunk ch = W.chunks[xcdChunk(blockIdx.x, W.nChunks)];
+    if (W.skip && W.skip[ch.scene]) return;
     const PbfConsts c = W.consts[ch.scene];
     body(c, ch);
     return;
@@ -96,6 +105,7 @@ template <typename Body> __device__ inline void forEachChunk(const Work& W, Body
     const uint32_t i = next;
     if (i >= W.nChunks) break;
     const Chunk ch = W.chunks[i];
+    if (W.skip && W.skip[ch.scene]) continue;
     const PbfConsts c = W.consts[ch.scene];
     body(c, ch);
   }
@@ -181,6 +191,17 @@ __device__ inline float spikyScale(const PbfConsts& c, float r) {
   if (!(r >= 1e-8f)) return 0.0f;
   const float q = c.h - r;
   return c.spiky * ((q * q) / r);
+}
+
+// sqrt and x / r: correctly rounded, or the hardware approximations (FAST).
+template <bool FAST> __device__ inline float sqrtM(float x) {
+  return FAST ? __builtin_amdgcn_sqrtf(x) : __builtin_sqrtf(x);
+}
+template <bool FAST> __device__ inline float spikyScaleM(const PbfConsts& c, float r) {
+  if (!FAST) return spikyScale(c, r);
+  if (!(r >= 1e-8f)) return 0.0f;
+  const float q = c.h - r;
+  return c.spiky * ((q * q) * __builtin_amdgcn_rcpf(r));
 }
 
 // 27-cell visit in the reference order (oclsph_kernel.h:150-192): x fastest,
@@ -315,6 +336,13 @@ struct Ctx {
   uint32_t persistentBlocks = 2048;        // grid of a persistent launch
   DevBuf<uint32_t> queues;                 // 2 counters per persistent launch slot
   uint32_t gridTotal = 0;                  // tableTotal of the last grid build
+  std::vector<uint32_t> chunkOffHost;      // first chunk of each scene, nScenes + 1 entries
+  DevBuf<uint32_t> chunkOff;
+  DevBuf<double> chunkSums;                // kSumWidth doubles per chunk
+  DevBuf<double> sceneStats;               // kStatSlots x kSumWidth doubles per scene
+  DevBuf<uint32_t> sceneDone;              // per scene: converged in this step
+  DevBuf<uint32_t> scenePasses;            // per scene: lambda passes run in the last step
+  bool converge = false;                   // some scene has density_tol > 0
   float mcScale = 1.0f;                    // world scale of the marching-cubes output
   DevBuf<uint32_t> localKey;               // scene-local Morton key per sorted particle
   int mcScene = 0;
@@ -427,12 +455,12 @@ inline dim3 gridOf(const Ctx* c, const Work& W) {
 
 // ---- launchers (one .hip file each) ----------------------------------------
 int prepareScenes(Ctx* c);                    // lf_api.hip: consts + chunks to the device
-Work workOf(Ctx* c, int queueSlot);           // queueSlot < 0: one workgroup per chunk
+Work workOf(Ctx* c, int queueSlot, bool skipDone = false);  // queueSlot < 0: one workgroup per chunk
 int gridBuild(Ctx* c);                        // lf_grid.hip
 int radixSort(Ctx* c, size_t n, int& outIdx);  // lf_sort.hip
 int pbfBuildLists(Ctx* c);                    // lf_pbf.hip
-int pbfLambda(Ctx* c);
-int pbfDelta(Ctx* c);
+int pbfLambda(Ctx* c, bool skipDone);  // skipDone: skip the scenes that converged
+int pbfDelta(Ctx* c, bool skipDone);
 int pbfFinalise(Ctx* c);
 int pbfXsph(Ctx* c);
 int pbfDiffuse(Ctx* c);
@@ -440,5 +468,12 @@ int mcSample(Ctx* c, const lf_mc_params& m);   // lf_mc.hip
 int mcClassifyCount(Ctx* c, const lf_mc_params& m);
 int mcEmit(Ctx* c, const lf_mc_params& m);
 int copyProbe(Ctx* c, size_t bytes, int reps, float* ms);  // lf_grid.hip
+// lf_diag.hip: FP64 per-scene reductions. densityCheck runs after lambda pass
+// `pass` (convergence); diagState reduces the current state into the last slot.
+constexpr int kSumWidth = 8;   // count, mass, momentum x3, sum |e|, max |e|, unused
+constexpr int kPassSlots = 16;
+constexpr int kStatSlots = kPassSlots + 1;
+int densityCheck(Ctx* c, int pass);
+int diagState(Ctx* c);
 
 }  // namespace lf

@@ -24,6 +24,7 @@
 // position, mass), V = (velocity, colour bits), T = type, P0 = clamped p* at
 // the list build.
 #include <algorithm>
+#include <type_traits>
 
 #include "lf_internal.hpp"
 
@@ -148,12 +149,12 @@ __device__ inline void neumaier(float& sum, float& comp, float term) {
 
 // One accepted pair of sph_lambda (oclsph_kernel.h:324-326): gradient sum
 // g += gradW / rho0 and density rho += m_a W6(r).
-template <bool KAHAN>
+template <bool KAHAN, bool FAST>
 __device__ inline void lambdaTerm(const PbfConsts& c, float ma, float dx, float dy, float dz,
                                   float d2, LambdaSums& A) {
-  const float r = __builtin_sqrtf(d2);
+  const float r = sqrtM<FAST>(d2);
   const float w = poly6W(c, r);
-  const float sc = spikyScale(c, r);
+  const float sc = spikyScaleM<FAST>(c, r);
   if (!KAHAN) {
     A.gx = __builtin_fmaf(dx * sc, c.rhoRecip, A.gx);
     A.gy = __builtin_fmaf(dy * sc, c.rhoRecip, A.gy);
@@ -210,7 +211,7 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
 // sph_lambda (oclsph_kernel.h:305-332). Rejected candidates contribute
 // exactly zero in the reference, so walking only the accepted ones gives the
 // same sums in the same order.
-template <int BS, bool KAHAN>
+template <int BS, bool KAHAN, bool FAST>
 __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restrict__ keys,
                                                const uint32_t* __restrict__ table,
                                                const uint8_t* __restrict__ T,
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
     forEachAccepted(
         c, L, 1, fresh != 0, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return P[b]; },
         [&](uint32_t, float4 pb, float d2) {
-          lambdaTerm<KAHAN>(c, pa.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, A);
+          lambdaTerm<KAHAN, FAST>(c, pa.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, A);
         });
     if (KAHAN) {
       A.gx += A.cgx;
@@ -246,13 +247,14 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
 }
 
 // One accepted pair of sph_delta (oclsph_kernel.h:355-358).
+template <bool FAST>
 __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float dx, float dy,
                                  float dz, float d2, float& ax, float& ay, float& az) {
-  const float r = __builtin_sqrtf(d2);
-  const float q = poly6W(c, r) / c.wdq;
+  const float r = sqrtM<FAST>(d2);
+  const float q = poly6W(c, r) * c.wdqRecip;
   const float corr = -c.corrK * ((q * q) * (q * q));
-  const float factor = ((la + lb) + corr) / c.rho0;
-  const float sc = spikyScale(c, r);
+  const float factor = ((la + lb) + corr) * c.rhoRecip;
+  const float sc = spikyScaleM<FAST>(c, r);
   ax = __builtin_fmaf(dx * sc, factor, ax);
   ay = __builtin_fmaf(dy * sc, factor, ay);
   az = __builtin_fmaf(dz * sc, factor, az);
@@ -262,7 +264,7 @@ __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float d
 // writes Pout. deltaP is not stored (nothing reads it after the pass). Also
 // raises D and the per-cell displacement to the new positions' (one atomic
 // per wave, and per run of equal keys).
-template <int BS>
+template <int BS, bool FAST>
 __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict__ keys,
                                               const uint32_t* __restrict__ table,
                                               const uint8_t* __restrict__ T,
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
             c, L, 2, fresh != 0, a, pa, keys[a], table + c.keyBase,
             [&](uint32_t b) { return PL[b]; },
             [&](uint32_t, float4 pb, float d2) {
-              deltaTerm(c, la, pb.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, ax, ay, az);
+              deltaTerm<FAST>(c, la, pb.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, ax, ay, az);
             });
         // pos = (p* + dp) * scale, clamped to the bounds (:364-371), p* = pos/scale
         const float acc[3] = {ax, ay, az};
@@ -445,8 +447,10 @@ __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict_
     forEachAccepted(
         c, L, 3, false, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return P[b]; },
         [&](uint32_t b, float4 pb, float d2) {
-          // P.w is the fluid mass, 0 for obstacles: they add zero-weight terms
-          const float mw = pb.w * poly6W(c, __builtin_sqrtf(d2));
+          // P.w is the fluid mass, 0 for obstacles: they add zero-weight terms.
+          // W6 on d2 (no square root).
+          const float t = c.hh - d2;
+          const float mw = pb.w * (c.poly6 * ((t * t) * t));
           const float4 vb = Vin[b];
           ax = __builtin_fmaf(mw, vb.x - va.x, ax);
           ay = __builtin_fmaf(mw, vb.y - va.y, ay);
@@ -473,31 +477,55 @@ int pbfBuildLists(Ctx* c) {
                         (const uint8_t*)c->T[c->cur].p, listsOf(c));
 }
 
-int pbfLambda(Ctx* c) {
-  if (c->n == 0) return LF_OK;
-  const Work W = workOf(c, kSlotLambda);
-  const int fresh = int(c->listFresh);
-  if (c->consts.flags & LF_FLAG_KAHAN) {
-    switch (c->chunkThreads) {
-      case 128: return launch(c, "pbf_lambda_kahan", k_lambda<128, true>, gridOf(c, W), dim3(128), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
-      case 512: return launch(c, "pbf_lambda_kahan", k_lambda<512, true>, gridOf(c, W), dim3(512), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
-      default: return launch(c, "pbf_lambda_kahan", k_lambda<256, true>, gridOf(c, W), dim3(256), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
-    }
-  }
+// Calls f with std::integral_constant arguments for the workgroup size and
+// the given flags, so one expression instantiates every kernel variant.
+template <typename F> int withBS(const Ctx* c, F&& f) {
   switch (c->chunkThreads) {
-    case 128: return launch(c, "pbf_lambda", k_lambda<128, false>, gridOf(c, W), dim3(128), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
-    case 512: return launch(c, "pbf_lambda", k_lambda<512, false>, gridOf(c, W), dim3(512), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
-    default: return launch(c, "pbf_lambda", k_lambda<256, false>, gridOf(c, W), dim3(256), 0, W, (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p, listsOf(c), fresh);
+    case 128: return f(std::integral_constant<int, 128>{});
+    case 512: return f(std::integral_constant<int, 512>{});
+    default: return f(std::integral_constant<int, 256>{});
   }
 }
+template <typename F> int withFlag(bool on, F&& f) {
+  return on ? f(std::true_type{}) : f(std::false_type{});
+}
 
-int pbfDelta(Ctx* c) {
+int pbfLambda(Ctx* c, bool skipDone) {
   if (c->n == 0) return LF_OK;
-  const Work W = workOf(c, kSlotDelta);
-  const int rc = LF_DISPATCH_BS(c, "pbf_delta", k_delta, W, (const uint32_t*)c->localKey.p,
-                                (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
-                                (const float4*)c->PL.p, (const float4*)c->X[c->cur].p,
-                                listsOf(c), int(c->listFresh), c->P.p);
+  const Work W = workOf(c, kSlotLambda, skipDone);
+  const int fresh = int(c->listFresh);
+  const uint32_t flags = c->consts.flags;
+  const char* name = (flags & LF_FLAG_KAHAN)
+                         ? ((flags & LF_FLAG_FAST_MATH) ? "pbf_lambda_kahan_fast" : "pbf_lambda_kahan")
+                         : ((flags & LF_FLAG_FAST_MATH) ? "pbf_lambda_fast" : "pbf_lambda");
+  return withBS(c, [&](auto bs) {
+    return withFlag(flags & LF_FLAG_KAHAN, [&](auto kahan) {
+      return withFlag(flags & LF_FLAG_FAST_MATH, [&](auto fast) {
+        return launch(c, name, k_lambda<decltype(bs)::value, decltype(kahan)::value,
+                                        decltype(fast)::value>,
+                      gridOf(c, W), dim3(decltype(bs)::value), 0, W,
+                      (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
+                      (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p,
+                      listsOf(c), fresh);
+      });
+    });
+  });
+}
+
+int pbfDelta(Ctx* c, bool skipDone) {
+  if (c->n == 0) return LF_OK;
+  const Work W = workOf(c, kSlotDelta, skipDone);
+  const bool fastMath = (c->consts.flags & LF_FLAG_FAST_MATH) != 0;
+  const int rc = withBS(c, [&](auto bs) {
+    return withFlag(fastMath, [&](auto fast) {
+      return launch(c, fastMath ? "pbf_delta_fast" : "pbf_delta",
+                    k_delta<decltype(bs)::value, decltype(fast)::value>, gridOf(c, W),
+                    dim3(decltype(bs)::value), 0, W, (const uint32_t*)c->localKey.p,
+                    (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
+                    (const float4*)c->PL.p, (const float4*)c->X[c->cur].p, listsOf(c),
+                    int(c->listFresh), c->P.p);
+    });
+  });
   c->listFresh = false;  // p* now differs from the predicted positions
   return rc;
 }

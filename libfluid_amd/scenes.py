@@ -79,13 +79,14 @@ class Scene:
     def n(self):
         return self.position.shape[0]
 
-    def params(self, flags=0, xsph_c=None, iterations=None):
+    def params(self, flags=0, xsph_c=None, iterations=None, density_tol=0.0):
         from . import pbf_params
         return pbf_params(self.h, dt=self.dt, scale=self.scale,
                           iterations=self.iterations if iterations is None else iterations,
                           gravity=self.gravity, min_bound=self.min_bound,
                           max_bound=self.max_bound,
-                          xsph_c=self.xsph_c if xsph_c is None else xsph_c, flags=flags)
+                          xsph_c=self.xsph_c if xsph_c is None else xsph_c, flags=flags,
+                          density_tol=density_tol)
 
     def mc(self, attributes=0):
         from . import mc_density_params

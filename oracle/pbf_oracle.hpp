@@ -21,6 +21,9 @@
  *  - W6(r)  = r <= h ? poly6 * t*t*t, t = fma(-r, r, h*h)  : 0
  *    gradW  = 1e-8 <= r <= h ? d * (spiky * ((h-r)*(h-r) / r)) : 0
  *    (oclsph_kernel.h:50-71);
+ *  - divisions by constants are multiplications by the folded reciprocal:
+ *    W6(r) / W6(dq) and the / rho0 of sph_delta (oclsph_kernel.h:354-358);
+ *  - XSPH and the density field take W6 on d2: t = h*h - d2;
  *  - neighbour candidates: the 27 cells around the particle's own cell, x
  *    fastest then y then z, each cell's particles in sorted order, no radius
  *    cut (oclsph_kernel.h:150-192);
@@ -93,7 +96,7 @@ template <typename F> inline uint32_t cellCoord(F q) {
 }
 
 template <typename F> struct Consts {
-  F h, hh, poly6, spiky, rhoRecip, wdq, invDt, minExt[3], maxExt[3], minB[3],
+  F h, hh, poly6, spiky, rhoRecip, wdq, wdqRecip, invDt, minExt[3], maxExt[3], minB[3],
       maxB[3], scale, dt, rho0, cfm, corrK, vd, xsph, xsphRho;
   uint32_t extent[3], G, iterations, flags;
   explicit Consts(const lf_pbf_params& p) {
@@ -107,6 +110,7 @@ template <typename F> struct Consts {
     rhoRecip = F(1) / rho0;
     const F dq = F(p.corr_dq) * h;
     wdq = W6(dq);
+    wdqRecip = F(1) / wdq;
     dt = F(p.dt);
     invDt = F(1) / dt;
     scale = F(p.scale);
@@ -355,9 +359,9 @@ template <typename F> class Pbf {
         const F* pb = &ps[4 * b];
         const F dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
         const F d = std::sqrt(dist2(dx, dy, dz));
-        const F q = c.W6(d) / c.wdq;
+        const F q = c.W6(d) * c.wdqRecip;
         const F corr = -c.corrK * ((q * q) * (q * q));
-        const F factor = ((la + lambda[b]) + corr) / c.rho0;
+        const F factor = ((la + lambda[b]) + corr) * c.rhoRecip;
         const F s = c.gradScale(d);
         acc[0] = std::fma(dx * s, factor, acc[0]);
         acc[1] = std::fma(dy * s, factor, acc[1]);
@@ -391,7 +395,7 @@ template <typename F> class Pbf {
   /* XSPH viscosity (absent in the reference; Macklin & Mueller 2013 eq. 17
    * with the SPH volume m_b / rho0 on each term, so c is a dimensionless
    * smoothing fraction): v_a += (c / rho0) sum_b m_b (v_b - v_a) W6(|p_a - p_b|)
-   * over fluid b, Jacobi. */
+   * over b within h (obstacles with zero weight), Jacobi. */
   void xsphPass() {
     if (c.xsph == F(0)) return;
     std::vector<F> out(vs);
@@ -405,10 +409,11 @@ template <typename F> class Pbf {
       forEachCandidate(keys[a], [&](uint32_t b) {
         const F* pb = &ps[4 * b];
         const F dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
-        const F d = std::sqrt(dist2(dx, dy, dz));
-        if (!(d <= c.h)) return;
-        // obstacles carry zero weight (their fluid mass is 0)
-        const F mw = (ts[b] == LF_OBSTACLE ? F(0) : ms[b]) * c.W6(d);
+        const F d2 = dist2(dx, dy, dz);
+        if (!(std::sqrt(d2) <= c.h)) return;
+        // W6 on d2; obstacles carry zero weight (their fluid mass is 0)
+        const F t = c.hh - d2;
+        const F mw = (ts[b] == LF_OBSTACLE ? F(0) : ms[b]) * (c.poly6 * ((t * t) * t));
         for (int k = 0; k < 3; ++k) acc[k] = std::fma(mw, vs[3 * b + k] - va[k], acc[k]);
       });
       for (int k = 0; k < 3; ++k) out[3 * a + k] = std::fma(c.xsphRho, acc[k], va[k]);
