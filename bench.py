@@ -76,17 +76,43 @@ def cpu_baseline(scene, mc, budget_s=20.0):
     return scene.n * steps / t, steps, threads
 
 
+MIXED_FLAGS = None  # set in main: FAST_MATH | KAHAN
+
+
+def _batched_ctx(lf, sc, flags, log, tag):
+    ctx = lf.Context(0)
+    ctx.batch_upload([s.params(flags=flags) for s in sc],
+                     [dict(position=s.position, velocity=s.velocity, mass=s.mass, type=s.type,
+                           colour=s.colour) for s in sc])
+    ctx.batch_step(1)
+    tune = ctx.autotune(3)
+    log("%s: autotune %s" % (tag, tune))
+    ctx.batch_step(1)
+    ctx.graph_capture(2)
+    return ctx, tune
+
+
+def _time_graph(ctx, steps, warmup):
+    ctx.graph_launch(max(1, warmup // 2))
+    ctx.sync()
+    reps = max(1, steps // 2)
+    t0 = time.perf_counter()
+    ctx.graph_launch(reps)
+    ctx.sync()
+    return 2 * reps, time.perf_counter() - t0
+
+
 def bench_batched(n_scenes, steps, warmup, seed=3, log=None):
-    """Aggregate particle-steps/s of a batch of uneven scenes, two ways:
-    plain = one context (own stream) per scene, stepped scene after scene
-    (one solver per scene, as libfluid runs); batched = all scenes in one
-    context, autotuned, whole steps replayed from a HIP graph."""
+    """Aggregate particle-steps/s of a batch of uneven scenes, three ways:
+    plain = one context (own stream) per scene, FP32, stepped scene after
+    scene (one solver per scene, as libfluid runs); batched = all scenes in one
+    context, autotuned, whole steps replayed from a HIP graph, FP32; batched
+    mixed = the same with FAST_MATH + KAHAN."""
     import libfluid_amd as lf
     from libfluid_amd import scenes
     sc = scenes.batch_scenes(n_scenes, seed=seed)
     total = sum(s.n for s in sc)
     log("batched: %d scenes, %d particles" % (len(sc), total))
-    # plain per-scene form
     ctxs = []
     for s in sc:
         c = lf.Context(0)
@@ -107,34 +133,60 @@ def bench_batched(n_scenes, steps, warmup, seed=3, log=None):
     for c, _ in ctxs:
         c.close()
     log("batched: plain per-scene %.3f s" % plain_s)
-    # batched form
-    ctx = lf.Context(0)
-    ctx.batch_upload([s.params() for s in sc],
-                     [dict(position=s.position, velocity=s.velocity, mass=s.mass, type=s.type,
-                           colour=s.colour) for s in sc])
-    ctx.batch_step(1)
-    tune = ctx.autotune(3)
-    log("batched: autotune %s" % tune)
-    ctx.batch_step(1)
-    ctx.graph_capture(2)
-    reps = max(1, warmup // 2)
-    ctx.graph_launch(reps)
+    out = {"workload": "%d scenes, %d particles (log-uniform %d..%d, seed %d), %d iterations"
+                       % (len(sc), total, min(s.n for s in sc), max(s.n for s in sc), seed,
+                          sc[0].iterations),
+           "unit": "particle-steps/s", "steps": steps,
+           "plain_per_scene_fp32": total * steps / plain_s}
+    for tag, flags in (("batched_graph_fp32", 0), ("batched_graph_mixed", MIXED_FLAGS)):
+        ctx, tune = _batched_ctx(lf, sc, flags, log, tag)
+        k, sec = _time_graph(ctx, steps, warmup)
+        ctx.close()
+        out[tag] = total * k / sec
+        out[tag + "_autotune"] = tune
+        log("%s: %d steps %.3f s" % (tag, k, sec))
+    out["speedup_batched_fp32"] = out["batched_graph_fp32"] / out["plain_per_scene_fp32"]
+    out["speedup_batched_mixed_vs_plain_fp32"] = (out["batched_graph_mixed"]
+                                                  / out["plain_per_scene_fp32"])
+    return out
+
+
+def bench_combined(steps, mc_steps, log):
+    """32 scenes totalling ~8M particles, mixed precision, 6 iterations, graph
+    replayed steps; then batched marching cubes at 512^3 per scene (FP32
+    density field, positions only) over every scene: particle-steps/s and
+    triangles/s."""
+    import libfluid_amd as lf
+    from libfluid_amd import scenes
+    sc = scenes.combined_scenes()
+    total = sum(s.n for s in sc)
+    log("combined: %d scenes, %d particles" % (len(sc), total))
+    ctx, tune = _batched_ctx(lf, sc, MIXED_FLAGS, log, "combined")
+    k, sec = _time_graph(ctx, steps, 2)
+    log("combined: %d steps %.3f s" % (k, sec))
+    ctx.mc_set_capacity(48 << 20)
+    m = sc[0].mc(attributes=0)
+    tris = 0
     ctx.sync()
-    reps = max(1, steps // 2)
     t0 = time.perf_counter()
-    ctx.graph_launch(reps)
+    for _ in range(mc_steps):
+        for s in range(len(sc)):
+            ctx.batch_select(s)
+            ctx.mc_sample(None, m)
+            ctx.mc_classify_count(m)
+            ctx.mc_emit(None, m)
+            tris += ctx.mc_count()
     ctx.sync()
-    batched_s = time.perf_counter() - t0
+    mc_sec = time.perf_counter() - t0
     ctx.close()
-    log("batched: graph %.3f s" % batched_s)
-    plain = total * steps / plain_s
-    batched = total * 2 * reps / batched_s
-    return {
-        "workload": "%d scenes, %d particles (log-uniform %d..%d, seed %d), FP32, %d iterations"
-                    % (len(sc), total, min(s.n for s in sc), max(s.n for s in sc), seed,
-                       sc[0].iterations),
-        "plain_per_scene": plain, "batched_graph": batched, "speedup": batched / plain,
-        "unit": "particle-steps/s", "steps": steps, "autotune": tune}
+    log("combined: marching cubes %d x %d scenes %.3f s, %d triangles" % (mc_steps, len(sc),
+                                                                        mc_sec, tris))
+    return {"workload": "%d scenes, %d particles (log-uniform 50k..500k rescaled to ~8M, seed 5), "
+                        "FAST_MATH+KAHAN, 6 iterations; marching cubes 512^3 per scene"
+                        % (len(sc), total),
+            "particle_steps_per_s": total * k / sec, "steps": k, "autotune": tune,
+            "mc_passes": mc_steps, "mc_samples_per_s": mc_steps * len(sc) * 512 ** 3 / mc_sec,
+            "triangles_per_s": tris / mc_sec, "triangles_per_pass": tris // mc_steps}
 
 
 def main():
@@ -146,6 +198,8 @@ def main():
     ap.add_argument("--scene", default="dam_break_1m")
     ap.add_argument("--batch-scenes", type=int, default=64,
                     help="scenes of the batched comparison (0: skip it)")
+    ap.add_argument("--no-combined", action="store_true",
+                    help="skip the 32-scene ~8M combined case")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -154,6 +208,8 @@ def main():
 
     import libfluid_amd as lf
     from libfluid_amd import scenes
+    global MIXED_FLAGS
+    MIXED_FLAGS = lf.LF_FLAG_FAST_MATH | lf.LF_FLAG_KAHAN
 
     scene = getattr(scenes, args.scene)()
     p = scene.params()
@@ -233,10 +289,33 @@ def main():
         "kernels": kernels,
         "device_ms_per_step": device_ms / args.steps,
     }
+    log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
+    # the same 1M workload with the mixed-precision options
+    pm = scene.params(flags=MIXED_FLAGS)
+    ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
+    ctx.profile(False)
+    for _ in range(args.warmup):
+        ctx.step(pm, 1)
+        ctx.mc_sample(pm, mc)
+        ctx.mc_classify_count(mc)
+        ctx.mc_emit(pm, mc)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.step(pm, 1)
+        ctx.mc_sample(pm, mc)
+        ctx.mc_classify_count(mc)
+        ctx.mc_emit(pm, mc)
+    ctx.sync()
+    mixed_s = time.perf_counter() - t0
+    out["mixed_1m"] = {"flags": "FAST_MATH|KAHAN", "value": n * args.steps / mixed_s,
+                       "ms_per_step": 1000.0 * mixed_s / args.steps}
+    log("mixed 1M: %.3f ms/step" % (1000.0 * mixed_s / args.steps))
     if args.batch_scenes > 0:
         out["batched"] = bench_batched(args.batch_scenes, max(4, args.steps // 5), args.warmup,
-                                       log=lambda m: print("bench: " + m, file=sys.stderr,
-                                                           flush=True))
+                                       log=log)
+    if not args.no_combined:
+        out["combined"] = bench_combined(max(4, args.steps // 10), 1, log)
     copy_bytes = 1 << 28
     copy_ms = ctx.copy_probe(copy_bytes, reps=20)
     out["copy_probe"] = {"bytes_moved": 2 * copy_bytes, "ms": copy_ms,

@@ -167,22 +167,26 @@ def _block_dims(n, aspect):
     return tuple(int(d) for d in dims)
 
 
-def batch_scenes(n_scenes=64, lo=2000, hi=200000, seed=3, iterations=4, spacing_range=(0.01, 0.05),
-                 box_factor=(2.0, 1.5, 1.25), mc_samples=(64, 64, 64)):
+def batch_scenes(n_scenes=64, lo=2000, hi=200000, seed=3, iterations=4, total=None,
+                 spacing_range=(0.01, 0.05), box_factor=(2.0, 1.5, 1.5), mc_samples=(64, 64, 64)):
     """Independent dam breaks of uneven size for batched stepping.
 
-    Particle counts are log-uniform in [lo, hi]; each scene is a jittered
-    lattice block of random aspect ratio (each axis x exp(U(-0.5, 0.5))) in the
-    corner of its own box (block extent x box_factor), with its own world
-    spacing (log-uniform in spacing_range) and so its own scale. Every scene
-    uses the solver-unit recipe of the module docstring (h = 2 spacings, about
-    33 neighbours within h)."""
+    Particle counts N_s = exp(U(ln lo, ln hi)) (seeded), optionally rescaled so
+    they sum to `total`; each scene is a jittered lattice block whose axes have
+    random aspect factors U(0.5, 2) normalised to N_s, in the corner of its own
+    box (block extent x box_factor: 2x in x, 1.5x in y and z), with its own
+    world spacing (log-uniform in spacing_range) and so its own scale. Every
+    scene uses the solver-unit recipe of the module docstring (h = 2 spacings,
+    about 33 neighbours within h)."""
     rng = np.random.default_rng(seed)
     counts = np.exp(rng.uniform(np.log(lo), np.log(hi), n_scenes))
+    if total is not None:
+        counts = counts * (float(total) / counts.sum())
     out = []
     for s in range(n_scenes):
-        aspect = np.exp(rng.uniform(-0.5, 0.5, 3))
-        dims = _block_dims(int(counts[s]), aspect)
+        aspect = rng.uniform(0.5, 2.0, 3)
+        aspect = aspect / np.cbrt(np.prod(aspect))
+        dims = _block_dims(int(round(counts[s])), aspect)
         sp = float(np.exp(rng.uniform(np.log(spacing_range[0]), np.log(spacing_range[1]))))
         sp = float(np.float32(sp))
         box = tuple(float(np.float32((d * sp) * f + 2 * sp)) for d, f in zip(dims, box_factor))
@@ -191,3 +195,11 @@ def batch_scenes(n_scenes=64, lo=2000, hi=200000, seed=3, iterations=4, spacing_
         sc.iterations = iterations
         out.append(sc)
     return out
+
+
+def combined_scenes(seed=5):
+    """32 scenes, N_s log-uniform in [50k, 500k] (seed 5) rescaled to a total
+    of about 8,000,000 particles, 6 solver iterations, marching cubes at 512^3
+    per scene."""
+    return batch_scenes(32, 50000, 500000, seed=seed, iterations=6, total=8000000,
+                        mc_samples=(512, 512, 512))
