@@ -129,11 +129,15 @@ int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string
 // cells instead; results are the same either way.
 constexpr uint32_t kMaxNbr = 128;
 
-// Slot of entry i of particle a's list: wave-chunked, entry-major within a
-// chunk of 64 particles, so the lanes of a wave reading entry i touch one
-// contiguous 256-byte run, and one wave's lists (24 KB) stay together.
+// Slot of entry i of particle a's list: groups of 8 entries (32 bytes) per
+// particle, the 64 particles of a wave side by side within a group, groups in
+// entry order. A wave reading entries i..i+7 of its 64 particles reads one
+// contiguous 2 KB run (two 16-byte loads per lane); the build stores a full
+// 32-byte group per lane at a time.
+constexpr uint32_t kNbrGroup = 8;
 __host__ __device__ inline uint64_t nbrSlot(uint32_t a, uint32_t i) {
-  return ((uint64_t(a >> 6) * kMaxNbr + i) << 6) | (a & 63u);
+  return ((uint64_t(a >> 6) * (kMaxNbr / kNbrGroup) + (i / kNbrGroup)) << 9) |
+         (uint64_t(a & 63u) << 3) | (i % kNbrGroup);
 }
 inline size_t nbrCapacity(size_t n) { return ((n + 63) / 64) * 64 * kMaxNbr; }
 

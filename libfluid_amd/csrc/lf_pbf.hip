@@ -31,37 +31,22 @@
 namespace lf {
 namespace {
 
-constexpr uint32_t kQueueCap = 24;  // LDS queue entries per lane
-constexpr uint32_t kDrainAt = 8;    // the wave drains once a lane holds this many
+#ifndef LF_BLOCK_LAMBDA
+#define LF_BLOCK_LAMBDA 8
+#endif
+#ifndef LF_BLOCK_DELTA
+#define LF_BLOCK_DELTA 8
+#endif
+#ifndef LF_BLOCK_XSPH
+#define LF_BLOCK_XSPH 8
+#endif
+constexpr int kBlockLambda = LF_BLOCK_LAMBDA;  // list entries in flight per lane (forEachAccepted)
+constexpr int kBlockDelta = LF_BLOCK_DELTA;
+constexpr int kBlockXsph = LF_BLOCK_XSPH;
 
-// Visits the candidates of a particle in the reference order (27 cells of
-// `key`) and passes those with accept(b) to use(b), in visit order, through a
-// per-lane LDS queue (q[i * BS]) that the whole wave drains together once
-// any lane holds kDrainAt entries: the stores of use() then issue as a few
-// wide instructions instead of one per candidate under divergence.
-template <int BS, typename Accept, typename Use>
-__device__ inline void visitCompacted(uint32_t key, const uint32_t* __restrict__ table, uint32_t G,
-                                      uint32_t* q, Accept&& accept, Use&& use) {
-  uint32_t qn = 0;
-  auto drain = [&]() {
-    for (uint32_t i = 0; i < qn; ++i) use(q[i * BS]);
-    qn = 0;
-  };
-  forEachNeighbourCell(key, table, G, [&](uint32_t s, uint32_t e) {
-    for (uint32_t b = s; b < e; ++b) {
-      if (accept(b)) {
-        if (qn == kQueueCap) drain();
-        q[qn * BS] = b;
-        ++qn;
-      }
-    }
-    if (__any(qn >= kDrainAt)) drain();
-  });
-  drain();
-}
 
 struct Lists {
-  uint32_t* nbr;           // nbrSlot(a, i)
+  uint32_t* nbr;           // nbrSlot(a, i), kMaxNbr entries per particle
   uint32_t* count;         // entries per particle (> kMaxNbr: overflowed)
   float4* P0;              // clamped p* at the build
   uint32_t* maxDisp;       // [0]: float bits of D, the largest |p* - P0| so far;
@@ -105,12 +90,13 @@ __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint3
 }
 
 // Neighbour lists of the predicted p* (see the file comment). P0 (the
-// clamped p*) was written by the reorder kernel.
+// clamped p*) was written by the reorder kernel. Each lane collects its
+// accepted candidates in an 8-entry register window and stores it as one
+// 32-byte group of its list whenever it fills.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ table,
                                                    const uint8_t* __restrict__ T, Lists L) {
-  __shared__ uint32_t queue[kQueueCap * BS];
   if (blockIdx.x == 0 && threadIdx.x < 4) L.maxDisp[threadIdx.x] = 0u;
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
@@ -121,16 +107,38 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
     }
     const float4 pa = L.P0[a];
     uint32_t cnt = 0;
-    visitCompacted<BS>(
-        keys[a], table + c.keyBase, c.G, queue + threadIdx.x,
-        [&](uint32_t b) {
-          const float4 pb = L.P0[b];
-          return dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z) <= c.listR2;
-        },
-        [&](uint32_t b) {
-          if (cnt < kMaxNbr) L.nbr[nbrSlot(a, cnt)] = b;
-          ++cnt;
-        });
+    uint32_t w[kNbrGroup];  // w[j]: entry (cnt - kNbrGroup + j) once shifted in
+#pragma unroll
+    for (int j = 0; j < int(kNbrGroup); ++j) w[j] = 0u;
+    forEachNeighbourCell(keys[a], table + c.keyBase, c.G, [&](uint32_t s, uint32_t e) {
+      for (uint32_t b = s; b < e; ++b) {
+        const float4 pb = L.P0[b];
+        if (!(dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z) <= c.listR2)) continue;
+#pragma unroll
+        for (int j = 0; j + 1 < int(kNbrGroup); ++j) w[j] = w[j + 1];
+        w[kNbrGroup - 1] = b;
+        ++cnt;
+        if (cnt % kNbrGroup == 0u && cnt <= kMaxNbr) {
+          uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, cnt - kNbrGroup));
+          g[0] = make_uint4(w[0], w[1], w[2], w[3]);
+          g[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        }
+      }
+    });
+    // the last r = cnt % 8 entries are w[8 - r] .. w[7]: shift them down to
+    // w[0] .. w[r - 1] (constant register indices) and store them
+    const uint32_t r = cnt % kNbrGroup;
+    if (r != 0u && cnt <= kMaxNbr) {
+#pragma unroll
+      for (uint32_t sh = 0; sh < kNbrGroup; ++sh)
+        if (sh < kNbrGroup - r)
+#pragma unroll
+          for (int j = 0; j + 1 < int(kNbrGroup); ++j) w[j] = w[j + 1];
+      uint32_t* g = L.nbr + nbrSlot(a, cnt - r);
+#pragma unroll
+      for (int j = 0; j < int(kNbrGroup); ++j)
+        if (uint32_t(j) < r) g[j] = w[j];
+    }
     L.count[a] = cnt;
   });
 }
@@ -168,10 +176,22 @@ __device__ inline void lambdaTerm(const PbfConsts& c, float ma, float dx, float 
   }
 }
 
+// Gathered neighbour records: a float4 whose xyz is p*, or position and
+// velocity together (XSPH).
+struct PosVel {
+  float4 p, v;
+};
+__device__ inline const float4& posOf(const float4& r) { return r; }
+__device__ inline const float4& posOf(const PosVel& r) { return r.p; }
+
 // Visits the accepted neighbours (r <= h) of particle a at position pa, in
 // the reference's order: through its list when usable, else the 27 cells.
-// load(b) returns the candidate's float4 (xyz = p*).
-template <typename Load, typename Use>
+// load(b) returns the candidate's float4 (xyz = p*). The list is walked in
+// blocks of BLK entries: BLK index loads, then BLK gathers, then the BLK pair
+// terms in list order, so a block costs one memory latency instead of BLK.
+// Entries past the count (the tail of the last block) are replaced by a itself
+// (a valid address) and skipped.
+template <int BLK, typename Load, typename Use>
 __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int pass, bool fresh,
                                        uint32_t a, float4 pa, uint32_t key,
                                        const uint32_t* __restrict__ table, Load&& load,
@@ -182,26 +202,34 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
   const uint64_t fell = __ballot(!usable);
   if (fell && __lane_id() == __builtin_ctzll(fell)) atomicAdd(&L.maxDisp[pass], uint32_t(__popcll(fell)));
   if (usable) {
-    uint32_t i = 0;
-    for (; i + 2 <= cnt; i += 2) {  // two gathers in flight
-      const uint32_t b0 = L.nbr[nbrSlot(a, i)], b1 = L.nbr[nbrSlot(a, i + 1)];
-      const float4 p0 = load(b0), p1 = load(b1);
-      const float d0 = dist2(pa.x - p0.x, pa.y - p0.y, pa.z - p0.z);
-      const float d1 = dist2(pa.x - p1.x, pa.y - p1.y, pa.z - p1.z);
-      if (d0 <= c.rmax2) use(b0, p0, d0);
-      if (d1 <= c.rmax2) use(b1, p1, d1);
-    }
-    if (i < cnt) {
-      const uint32_t b0 = L.nbr[nbrSlot(a, i)];
-      const float4 p0 = load(b0);
-      const float d0 = dist2(pa.x - p0.x, pa.y - p0.y, pa.z - p0.z);
-      if (d0 <= c.rmax2) use(b0, p0, d0);
+    static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
+    for (uint32_t i = 0; i < cnt; i += BLK) {
+      uint32_t b[BLK];
+#pragma unroll
+      for (int k = 0; k < BLK; k += 4) {  // 16-byte loads; entries past cnt read as a
+        const uint4 q = *reinterpret_cast<const uint4*>(L.nbr + nbrSlot(a, i + k));
+        b[k] = q.x;
+        b[k + 1] = (i + k + 1 < cnt) ? q.y : a;
+        b[k + 2] = (i + k + 2 < cnt) ? q.z : a;
+        b[k + 3] = (i + k + 3 < cnt) ? q.w : a;
+        if (!(i + k < cnt)) b[k] = a;
+      }
+      decltype(load(0u)) p[BLK];
+#pragma unroll
+      for (int k = 0; k < BLK; ++k) p[k] = load(b[k]);
+#pragma unroll
+      for (int k = 0; k < BLK; ++k) {
+        const float4& q = posOf(p[k]);
+        const float d2 = dist2(pa.x - q.x, pa.y - q.y, pa.z - q.z);
+        if (i + k < cnt && d2 <= c.rmax2) use(b[k], p[k], d2);
+      }
     }
   } else {
     forEachNeighbourCell(key, table, c.G, [&](uint32_t s, uint32_t e) {
       for (uint32_t b = s; b < e; ++b) {
-        const float4 pb = load(b);
-        const float d2 = dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z);
+        const auto pb = load(b);
+        const float4& q = posOf(pb);
+        const float d2 = dist2(pa.x - q.x, pa.y - q.y, pa.z - q.z);
         if (d2 <= c.rmax2) use(b, pb, d2);
       }
     });
@@ -228,7 +256,7 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
       return;
     }
     LambdaSums A;
-    forEachAccepted(
+    forEachAccepted<kBlockLambda>(
         c, L, 1, fresh != 0, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return P[b]; },
         [&](uint32_t, float4 pb, float d2) {
           lambdaTerm<KAHAN, FAST>(c, pa.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, A);
@@ -284,7 +312,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
       } else {
         const float la = pa.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        forEachAccepted(
+        forEachAccepted<kBlockDelta>(
             c, L, 2, fresh != 0, a, pa, keys[a], table + c.keyBase,
             [&](uint32_t b) { return PL[b]; },
             [&](uint32_t, float4 pb, float d2) {
@@ -444,14 +472,16 @@ __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict_
     }
     const float4 pa = P[a];
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    forEachAccepted(
-        c, L, 3, false, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return P[b]; },
-        [&](uint32_t b, float4 pb, float d2) {
+    forEachAccepted<kBlockXsph>(
+        c, L, 3, false, a, pa, keys[a], table + c.keyBase,
+        [&](uint32_t b) { return PosVel{P[b], Vin[b]}; },
+        [&](uint32_t, const PosVel& r, float d2) {
+          const float4& pb = r.p;
           // P.w is the fluid mass, 0 for obstacles: they add zero-weight terms.
           // W6 on d2 (no square root).
           const float t = c.hh - d2;
           const float mw = pb.w * (c.poly6 * ((t * t) * t));
-          const float4 vb = Vin[b];
+          const float4& vb = r.v;
           ax = __builtin_fmaf(mw, vb.x - va.x, ax);
           ay = __builtin_fmaf(mw, vb.y - va.y, ay);
           az = __builtin_fmaf(mw, vb.z - va.z, az);
