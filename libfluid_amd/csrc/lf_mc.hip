@@ -79,7 +79,9 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
                                                          const float4* __restrict__ P,
                                                          float* __restrict__ field,
                                                          uint32_t tilesY, uint32_t tilesZ) {
-  __shared__ float4 stage[kStageCap];
+  // staged particles as separate x, y, z, m arrays: lanes reading different
+  // particles then hit different banks far more often than with 16-byte records
+  __shared__ float stX[kStageCap], stY[kStageCap], stZ[kStageCap], stM[kStageCap];
   __shared__ uint32_t cellStart[kHaloCells + 1];
   __shared__ uint32_t scanTmp[kThreads / 64];
   __shared__ int mode;  // 0 LDS, 1 global
@@ -168,7 +170,13 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
         const uint32_t ix = h % hx, iy = (h / hx) % hy, iz = h / (hx * hy);
         const uint32_t code = mortonEncode(lo[0] - 1 + ix, lo[1] - 1 + iy, lo[2] - 1 + iz);
         const uint32_t s0 = table[code], dst = cellStart[h];
-        for (uint32_t j = 0; j < myCount[r]; ++j) stage[dst + j] = P[s0 + j];
+        for (uint32_t j = 0; j < myCount[r]; ++j) {
+          const float4 q = P[s0 + j];
+          stX[dst + j] = q.x;
+          stY[dst + j] = q.y;
+          stZ[dst + j] = q.z;
+          stM[dst + j] = q.w;
+        }
       }
     }
   }
@@ -192,12 +200,12 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
     const uint32_t bx = cx - lo[0], by = cy - lo[1], bz = cz - lo[2];  // halo index of dx=-1
     for (uint32_t dz = 0; dz < 3; ++dz)
       for (uint32_t dy = 0; dy < 3; ++dy) {
+        // the three x cells of a row are consecutive halo cells, so their
+        // particles are one contiguous run of the staging, in visit order
         const uint32_t row = ((bz + dz) * hy + (by + dy)) * hx + bx;
-        for (uint32_t dx = 0; dx < 3; ++dx) {
-          const uint32_t e = cellStart[row + dx + 1];
-          for (uint32_t b = cellStart[row + dx]; b < e; ++b)
-            rho += densityTerm(c, stage[b], px, py, pz);
-        }
+        const uint32_t e = cellStart[row + 3];
+        for (uint32_t b = cellStart[row]; b < e; ++b)
+          rho += densityTerm(c, make_float4(stX[b], stY[b], stZ[b], stM[b]), px, py, pz);
       }
   } else {
     forEachNeighbourCell(cellKey(c, px, py, pz), table, c.G, [&](uint32_t s, uint32_t e) {

@@ -43,6 +43,10 @@ namespace {
 constexpr int kBlockLambda = LF_BLOCK_LAMBDA;  // list entries in flight per lane (forEachAccepted)
 constexpr int kBlockDelta = LF_BLOCK_DELTA;
 constexpr int kBlockXsph = LF_BLOCK_XSPH;
+#ifndef LF_BLOCK_BUILD
+#define LF_BLOCK_BUILD 1
+#endif
+constexpr int kBuildBlock = LF_BLOCK_BUILD;  // candidates loaded together (list build)
 
 
 struct Lists {
@@ -111,17 +115,24 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
 #pragma unroll
     for (int j = 0; j < int(kNbrGroup); ++j) w[j] = 0u;
     forEachNeighbourCell(keys[a], table + c.keyBase, c.G, [&](uint32_t s, uint32_t e) {
-      for (uint32_t b = s; b < e; ++b) {
-        const float4 pb = L.P0[b];
-        if (!(dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z) <= c.listR2)) continue;
+      for (uint32_t b0 = s; b0 < e; b0 += kBuildBlock) {
+        float4 pb[kBuildBlock];  // candidates loaded together (indices clamped into the cell)
 #pragma unroll
-        for (int j = 0; j + 1 < int(kNbrGroup); ++j) w[j] = w[j + 1];
-        w[kNbrGroup - 1] = b;
-        ++cnt;
-        if (cnt % kNbrGroup == 0u && cnt <= kMaxNbr) {
-          uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, cnt - kNbrGroup));
-          g[0] = make_uint4(w[0], w[1], w[2], w[3]);
-          g[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        for (int k = 0; k < kBuildBlock; ++k) pb[k] = L.P0[min(b0 + k, e - 1u)];
+#pragma unroll
+        for (int k = 0; k < kBuildBlock; ++k) {
+          const uint32_t b = b0 + k;
+          if (!(b < e) || !(dist2(pa.x - pb[k].x, pa.y - pb[k].y, pa.z - pb[k].z) <= c.listR2))
+            continue;
+#pragma unroll
+          for (int j = 0; j + 1 < int(kNbrGroup); ++j) w[j] = w[j + 1];
+          w[kNbrGroup - 1] = b;
+          ++cnt;
+          if (cnt % kNbrGroup == 0u && cnt <= kMaxNbr) {
+            uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, cnt - kNbrGroup));
+            g[0] = make_uint4(w[0], w[1], w[2], w[3]);
+            g[1] = make_uint4(w[4], w[5], w[6], w[7]);
+          }
         }
       }
     });
