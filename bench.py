@@ -52,6 +52,69 @@ def algorithmic_bytes(name, n, G, S, T):
     return table.get(name)
 
 
+# The kernel (rocprof name prefix) behind each launch name of the library.
+KERNEL_SYMBOL = {
+    "predict_key": "k_predict_key", "radix_hist": "k_radix_hist", "radix_scan": "k_radix_scan",
+    "radix_scatter": "k_radix_scatter", "reorder": "k_reorder", "cell_table": "k_cell_table",
+    "pbf_build_list": "k_build_list", "pbf_lambda": "k_lambda", "pbf_lambda_kahan": "k_lambda",
+    "pbf_delta": "k_delta", "pbf_finalise": "k_finalise", "pbf_xsph": "k_xsph",
+    "pbf_diffuse": "k_diffuse", "mc_density_field": "k_mc_density", "mc_classify_count": "k_mc_count",
+    "mc_scan": "k_mc_scan", "mc_emit": "k_mc_emit", "copy_probe": "k_copy_probe",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_dam_break_1m.txt")
+
+
+def measured_traffic():
+    """HBM bytes per dispatch by kernel symbol, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
+    calibration); {} when absent."""
+    out = {}
+    if not os.path.exists(TRAFFIC_FILE):
+        return out
+    for line in open(TRAFFIC_FILE):
+        f = line.split("|")
+        if len(f) < 6 or not f[1].strip().startswith("k_"):
+            continue
+        try:
+            out[f[1].strip()] = int(f[4].strip())
+        except ValueError:
+            pass
+    return out
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_bytes=1 << 28):
+    """Per-kernel mean FETCH_SIZE / WRITE_SIZE per dispatch (KiB) from two
+    rocprofv3 --pmc runs, scaled so the copy-probe kernel (copy_bytes read,
+    copy_bytes written per dispatch) reads exactly its size. Prints a
+    markdown table: kernel | fetch KiB | write KiB | HBM bytes | factors."""
+    import csv
+    import glob
+    import re
+
+    def load(d, counter):
+        acc = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] != counter:
+                    continue
+                m = re.search(r"(k_[a-z_0-9]+(<[^>]*>)?)\(", r["Kernel_Name"])
+                if not m:
+                    continue
+                acc.setdefault(m.group(1), []).append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    kib = copy_bytes / 1024.0
+    ff = kib / fetch["k_copy_probe"] if fetch.get("k_copy_probe") else 1.0
+    fw = kib / write["k_copy_probe"] if write.get("k_copy_probe") else 1.0
+    print("| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes per dispatch | scale |")
+    print("|---|---:|---:|---:|---|")
+    for k in sorted(set(fetch) | set(write)):
+        fb, wb = fetch.get(k, 0.0), write.get(k, 0.0)
+        total = int(round((fb * ff + wb * fw) * 1024))
+        print("| %s | %.0f | %.0f | %d | fetch x%.3f, write x%.3f |" % (k, fb, wb, total, ff, fw))
+
+
 def cpu_baseline(scene, mc, budget_s=20.0):
     """The CPU restatement (FP32, OpenMP) on the first steps of the scene,
     marching cubes included; returns (particle-steps/s, steps, threads)."""
@@ -200,7 +263,12 @@ def main():
                     help="scenes of the batched comparison (0: skip it)")
     ap.add_argument("--no-combined", action="store_true",
                     help="skip the 32-scene ~8M combined case")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print the per-kernel HBM traffic table of two rocprofv3 --pmc runs")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -246,6 +314,13 @@ def main():
     G = ctx.grid_info()[0]
     S = int(np.prod(mc.samples))
 
+    traffic = measured_traffic()
+
+    def traffic_of(name):
+        sym = KERNEL_SYMBOL.get(name)
+        hits = [v for k, v in traffic.items() if sym and (k == sym or k.startswith(sym + "<"))]
+        return hits[0] if hits else None
+
     value = n * args.steps / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
     device_ms = sum(v[1] for v in stats.values())
@@ -259,7 +334,9 @@ def main():
         avg = ms / cnt / 1000.0
         kernels[name] = {"launches": cnt, "avg_us": round(avg * 1e6, 2),
                          "share": round(ms / device_ms, 4),
-                         "alg_GBs": round(b / avg / 1e9, 1) if b else None}
+                         "alg_GBs": round(b / avg / 1e9, 1) if b else None,
+                         "hbm_GBs": (round(traffic_of(name) / avg / 1e9, 1)
+                                     if traffic_of(name) else None)}
 
     out = {
         "metric": "PBF particle-steps/s @1M, 4 iters (% HBM peak); batched-mixed vs plain FP32",
@@ -284,7 +361,7 @@ def main():
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "alg_bytes_per_launch": dom_bytes,
+                     "traffic": traffic_of(dom_name), "alg_bytes_per_launch": dom_bytes,
                      "avg_launch_us": round(dom_avg_s * 1e6, 2)},
         "kernels": kernels,
         "device_ms_per_step": device_ms / args.steps,
