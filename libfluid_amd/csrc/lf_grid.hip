@@ -81,21 +81,24 @@ __device__ inline uint32_t lowerBound(const uint32_t* __restrict__ keys, uint32_
   return lo;
 }
 
-constexpr uint32_t kCells = 4096;            // cells per workgroup
-constexpr uint32_t kPerThread = kCells / kThreads;
+constexpr uint32_t kCells = 1024;            // cells per workgroup
+constexpr uint32_t kRounds = kCells / kThreads;
+constexpr uint32_t kMarkUnroll = 4;          // keys in flight per thread while marking
 
 // K4: table[c] = first sorted index whose key >= c, for c in [0, G) (G: the
-// concatenated tables of all scenes): the
-// monotone scan of oclsph.hpp:758-765 as a per-tile lower bound. Each
-// workgroup owns kCells cells: segment starts of its key range are marked in
-// LDS, then a suffix-min fills empty cells with the next start.
+// concatenated tables of all scenes): the monotone scan of oclsph.hpp:758-765
+// as a per-tile lower bound. Each workgroup owns kCells cells: the run starts
+// of its key range are marked in LDS, then a suffix minimum fills each empty
+// cell with the next start. The suffix minimum runs over rounds of 256 cells,
+// last round first, each a wave-shuffle suffix minimum plus the minima of the
+// later waves and rounds, so LDS reads and table stores are consecutive.
 __global__ __launch_bounds__(kThreads) void k_cell_table(const uint32_t* __restrict__ keys,
                                                          uint32_t n, uint32_t G,
                                                          uint32_t* __restrict__ table,
                                                          uint32_t* __restrict__ cellDisp) {
   __shared__ uint32_t local[kCells];
   __shared__ uint32_t range[2];
-  __shared__ uint32_t thrMin[kThreads];
+  __shared__ uint32_t waveMin[kThreads / 64];
   const uint32_t c0 = blockIdx.x * kCells;
   const uint32_t c1 = min(c0 + kCells, G);
   if (threadIdx.x < 2) range[threadIdx.x] = lowerBound(keys, n, threadIdx.x == 0 ? c0 : c1);
@@ -103,32 +106,44 @@ __global__ __launch_bounds__(kThreads) void k_cell_table(const uint32_t* __restr
   const uint32_t s0 = range[0], s1 = range[1];
   for (uint32_t i = threadIdx.x; i < kCells; i += kThreads) local[i] = s1;
   __syncthreads();
-  for (uint32_t s = s0 + threadIdx.x; s < s1; s += kThreads) {
-    const uint32_t k = keys[s];
-    if ((s == 0 || keys[s - 1] != k) && k - c0 < kCells) local[k - c0] = s;
+  // run starts; a dense tile holds ~8 particles per cell, so each thread
+  // walks its strided share with kMarkUnroll key pairs in flight
+  for (uint32_t base = s0; base < s1; base += kThreads * kMarkUnroll) {
+    uint32_t k[kMarkUnroll], kp[kMarkUnroll];
+#pragma unroll
+    for (uint32_t u = 0; u < kMarkUnroll; ++u) {
+      const uint32_t s = min(base + u * kThreads + threadIdx.x, s1 - 1u);
+      k[u] = keys[s];
+      kp[u] = s > 0 ? keys[s - 1] : ~k[u];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kMarkUnroll; ++u) {
+      const uint32_t s = base + u * kThreads + threadIdx.x;
+      if (s < s1 && kp[u] != k[u] && k[u] - c0 < kCells) local[k[u] - c0] = s;
+    }
   }
   __syncthreads();
-  // suffix minimum over local[0, kCells): thread t owns [t*16, t*16+16)
-  const uint32_t lo = threadIdx.x * kPerThread;
-  uint32_t v[kPerThread];
-  uint32_t m = s1;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t carry = s1;  // minimum over the cells of the rounds already done
+  for (int r = int(kRounds) - 1; r >= 0; --r) {
+    const uint32_t i = uint32_t(r) * kThreads + threadIdx.x;
+    uint32_t v = local[i];
 #pragma unroll
-  for (int j = int(kPerThread) - 1; j >= 0; --j) {
-    m = min(m, local[lo + j]);
-    v[j] = m;
-  }
-  thrMin[threadIdx.x] = m;
-  __syncthreads();
-  // suffix min of the thread minima (simple: each thread scans those after it)
-  uint32_t after = s1;
-  for (uint32_t t = threadIdx.x + 1; t < kThreads; ++t) after = min(after, thrMin[t]);
-#pragma unroll
-  for (uint32_t j = 0; j < kPerThread; ++j) {
-    const uint32_t cell = c0 + lo + j;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t o = __shfl_down(v, off, 64);
+      if (lane + off < 64) v = min(v, o);
+    }
+    if (lane == 0) waveMin[w] = v;  // lane 0 holds its wave's minimum
+    __syncthreads();
+    uint32_t later = carry;
+    for (uint32_t x = w + 1; x < kThreads / 64; ++x) later = min(later, waveMin[x]);
+    const uint32_t cell = c0 + i;
     if (cell < c1) {
-      table[cell] = min(v[j], after);
+      table[cell] = min(v, later);
       cellDisp[cell] = 0u;  // per-cell displacement of the neighbour lists
     }
+    for (uint32_t x = 0; x < kThreads / 64; ++x) carry = min(carry, waveMin[x]);
+    __syncthreads();
   }
 }
 
