@@ -298,18 +298,39 @@ __global__ __launch_bounds__(kThreads) void k_mc_reference(
 struct CubeGrid {
   uint32_t S[3];
   uint32_t M[3];
-  uint64_t cubes;
+  uint32_t cubes;  // < 2^31 (checked on the host)
 };
 
+// Cube m = (x*My + y)*Mz + z (to3d, curves.h:22-27). The kernels walk cubes
+// m0, m0 + kThreads, ... : the coordinates are divided out once, then stepped.
+struct CubeCursor {
+  uint32_t x, y, z;
+  __device__ CubeCursor(const CubeGrid& g, uint32_t m) {
+    z = m % g.M[2];
+    const uint32_t xy = m / g.M[2];
+    y = xy % g.M[1];
+    x = xy / g.M[1];
+  }
+  __device__ void advance(const CubeGrid& g, uint32_t by) {
+    z += by;
+    while (z >= g.M[2]) {
+      z -= g.M[2];
+      if (++y == g.M[1]) {
+        y = 0;
+        ++x;
+      }
+    }
+  }
+};
+
+__device__ inline uint64_t sampleIndex(const CubeGrid& g, uint32_t x, uint32_t y, uint32_t z) {
+  return (uint64_t(x) * g.S[1] + y) * g.S[2] + z;
+}
+
 __device__ inline uint32_t cubeCase(const CubeGrid& g, const float* __restrict__ field,
-                                    uint64_t m, float iso, uint32_t& x, uint32_t& y,
-                                    uint32_t& z) {
-  z = uint32_t(m % g.M[2]);
-  const uint64_t xy = m / g.M[2];
-  y = uint32_t(xy % g.M[1]);
-  x = uint32_t(xy / g.M[1]);
+                                    const CubeCursor& q, float iso) {
   const uint64_t sz = g.S[2], syz = uint64_t(g.S[1]) * g.S[2];
-  const uint64_t b = uint64_t(x) * syz + uint64_t(y) * sz + z;
+  const uint64_t b = sampleIndex(g, q.x, q.y, q.z);
   uint32_t ci = 0;
   const float v0 = field[b], v1 = field[b + syz], v2 = field[b + syz + sz], v3 = field[b + sz];
   const float v4 = field[b + 1], v5 = field[b + syz + 1], v6 = field[b + syz + sz + 1],
@@ -332,13 +353,14 @@ __global__ __launch_bounds__(kThreads) void k_mc_count(CubeGrid g, const float* 
                                                        uint8_t* __restrict__ cases) {
   __shared__ uint32_t sm[kThreads / 64];
   uint32_t sum = 0;
-  const uint64_t base = uint64_t(blockIdx.x) * kCubesPerBlock;
+  const uint32_t m0 = blockIdx.x * kCubesPerBlock + threadIdx.x;
+  CubeCursor q(g, min(m0, g.cubes - 1u));
 #pragma unroll
   for (int j = 0; j < kCubesPerThread; ++j) {
-    const uint64_t m = base + uint64_t(j) * kThreads + threadIdx.x;
+    const uint32_t m = m0 + uint32_t(j) * kThreads;
+    if (j > 0) q.advance(g, kThreads);
     if (m < g.cubes) {
-      uint32_t x, y, z;
-      const uint32_t ci = cubeCase(g, field, m, iso, x, y, z);
+      const uint32_t ci = cubeCase(g, field, q, iso);
       if (cases) cases[m] = uint8_t(ci);
       sum += cTab.ntri[ci];
     }
@@ -421,97 +443,99 @@ __device__ inline uint32_t blockPrefixSmall(uint32_t v, uint32_t* sm, uint32_t* 
   return wb + pre;
 }
 
+// One vertex of edge e of the cube at q: t = (iso - v_a)/(v_b - v_a),
+// position mix(p_a, p_b, t) with p = (origin + o*step)*scale, and with
+// attributes the normal and colour mixed the same way.
+struct EdgeVertex {
+  float p[3];
+  float t;
+  uint64_t ia, ib;  // sample indices of the edge's corners
+};
+__device__ inline EdgeVertex edgeVertex(const EmitArgs& A, const float* __restrict__ field,
+                                        const CubeCursor& q, int e) {
+  EdgeVertex r;
+  const int a = cEdge[e][0], b = cEdge[e][1];
+  const uint32_t oa[3] = {q.x + uint32_t(cCorner[a][0]), q.y + uint32_t(cCorner[a][1]),
+                          q.z + uint32_t(cCorner[a][2])};
+  const uint32_t ob[3] = {q.x + uint32_t(cCorner[b][0]), q.y + uint32_t(cCorner[b][1]),
+                          q.z + uint32_t(cCorner[b][2])};
+  r.ia = sampleIndex(A.g, oa[0], oa[1], oa[2]);
+  r.ib = sampleIndex(A.g, ob[0], ob[1], ob[2]);
+  const float va = field[r.ia], vb = field[r.ib];
+  r.t = (A.iso - va) / (vb - va);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float pa = __builtin_fmaf(float(oa[k]), A.step[k], A.origin[k]) * A.scale;
+    const float pb = __builtin_fmaf(float(ob[k]), A.step[k], A.origin[k]) * A.scale;
+    r.p[k] = __builtin_fmaf(pb - pa, r.t, pa);
+  }
+  return r;
+}
+
 // mc_eval (oclsph_kernel.h:599-735): edge intersections by linear
 // interpolation t = (iso - v0)/(v1 - v0), mix(p0, p1, t); triangles of
-// TriTable[ci] written at the scanned offset.
+// TriTable[ci] written at the scanned offset. Blocks without triangles exit
+// at once; each emitted vertex is interpolated where it is written.
 __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* __restrict__ field,
                                                       const float4* __restrict__ normals,
                                                       const uint32_t* __restrict__ colours,
                                                       const uint32_t* __restrict__ blockOffset,
+                                                      const uint32_t* __restrict__ total,
+                                                      uint32_t nBlocks,
                                                       float* __restrict__ triPos,
                                                       float* __restrict__ triNrm,
                                                       uint32_t* __restrict__ triCol) {
   __shared__ uint32_t sm[kThreads / 64];
-  const uint64_t base = uint64_t(blockIdx.x) * kCubesPerBlock;
-  uint64_t run = blockOffset[blockIdx.x];
+  const uint64_t first = blockOffset[blockIdx.x];
+  const uint64_t next = (blockIdx.x + 1 < nBlocks) ? blockOffset[blockIdx.x + 1] : total[0];
+  if (next == first) return;  // uniform: no triangle in this block
+  uint64_t run = first;
+  const uint32_t m0 = blockIdx.x * kCubesPerBlock + threadIdx.x;
+  CubeCursor q(A.g, min(m0, A.g.cubes - 1u));
   for (int j = 0; j < kCubesPerThread; ++j) {
-    const uint64_t m = base + uint64_t(j) * kThreads + threadIdx.x;
-    uint32_t ci = 0, x = 0, y = 0, z = 0;
-    if (m < A.g.cubes) ci = cubeCase(A.g, field, m, A.iso, x, y, z);
+    const uint32_t m = m0 + uint32_t(j) * kThreads;
+    if (j > 0) q.advance(A.g, kThreads);
+    uint32_t ci = 0;
+    if (m < A.g.cubes) ci = cubeCase(A.g, field, q, A.iso);
     const uint32_t nt = cTab.ntri[ci];
     uint32_t tot;
     const uint32_t pre = blockPrefixSmall(nt, sm, &tot);
-    if (nt > 0) {
-      const uint64_t off = run + pre;
-      const uint64_t sz = A.g.S[2], syz = uint64_t(A.g.S[1]) * A.g.S[2];
-      float val[8], pos[8][3];
-      uint64_t sidx[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const uint32_t o[3] = {x + uint32_t(cCorner[i][0]), y + uint32_t(cCorner[i][1]),
-                               z + uint32_t(cCorner[i][2])};
-        sidx[i] = uint64_t(o[0]) * syz + uint64_t(o[1]) * sz + o[2];
-        val[i] = field[sidx[i]];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-          pos[i][k] = __builtin_fmaf(float(o[k]), A.step[k], A.origin[k]) * A.scale;
-      }
-      const uint32_t edges = cTab.edge[ci];
-      float ts[12][3];
-      float tval[12];
-#pragma unroll
-      for (int e = 0; e < 12; ++e) {
-        tval[e] = 0.0f;
-        if (edges & (1u << e)) {
-          const int a = cEdge[e][0], b = cEdge[e][1];
-          const float t = (A.iso - val[a]) / (val[b] - val[a]);
-          tval[e] = t;
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-            ts[e][k] = __builtin_fmaf(pos[b][k] - pos[a][k], t, pos[a][k]);
-        } else {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) ts[e][k] = 0.0f;
-        }
-      }
-      for (uint32_t i = 0; i < nt; ++i) {
-        const uint64_t slot = off + i;
-        if (slot >= A.cap) break;
-        float* dst = triPos + slot * 9;
-        for (int v = 0; v < 3; ++v) {
-          const int e = cTab.tri[ci][3 * i + v];
-          dst[3 * v + 0] = ts[e][0];
-          dst[3 * v + 1] = ts[e][1];
-          dst[3 * v + 2] = ts[e][2];
-          if (A.attributes) {
-            const int a = cEdge[e][0], b = cEdge[e][1];
-            const float t = tval[e];
-            float na[3] = {0.0f, 0.0f, 0.0f}, nb[3] = {0.0f, 0.0f, 0.0f};
-            float ca[4], cb[4];
-            if (normals) {
-              const float4 n0 = normals[sidx[a]], n1 = normals[sidx[b]];
-              na[0] = n0.x; na[1] = n0.y; na[2] = n0.z;
-              nb[0] = n1.x; nb[1] = n1.y; nb[2] = n1.z;
-            }
-            if (colours) {
-              const uint32_t c0 = colours[sidx[a]], c1 = colours[sidx[b]];
-              for (int q = 0; q < 4; ++q) {
-                ca[q] = float((c0 >> (24 - 8 * q)) & 0xFFu);
-                cb[q] = float((c1 >> (24 - 8 * q)) & 0xFFu);
-              }
-            } else {
-              for (int q = 0; q < 4; ++q) ca[q] = cb[q] = 255.0f;
-            }
-            float* nd = triNrm + slot * 9;
-            for (int k = 0; k < 3; ++k) nd[3 * v + k] = __builtin_fmaf(nb[k] - na[k], t, na[k]);
-            uint32_t packed = 0;
-            for (int q = 0; q < 4; ++q) {
-              const float mv = __builtin_fmaf(cb[q] - ca[q], t, ca[q]);
-              const uint32_t u = !(mv > 0.0f) ? 0u : (mv >= 255.0f ? 255u : uint32_t(mv));
-              packed = (packed << 8) | u;
-            }
-            triCol[slot * 3 + v] = packed;
+    for (uint32_t i = 0; i < nt; ++i) {
+      const uint64_t slot = run + pre + i;
+      if (slot >= A.cap) break;
+      for (int v = 0; v < 3; ++v) {
+        const int e = cTab.tri[ci][3 * i + v];
+        const EdgeVertex r = edgeVertex(A, field, q, e);
+        float* dst = triPos + slot * 9 + 3 * v;
+        dst[0] = r.p[0];
+        dst[1] = r.p[1];
+        dst[2] = r.p[2];
+        if (A.attributes) {
+          float na[3] = {0.0f, 0.0f, 0.0f}, nb[3] = {0.0f, 0.0f, 0.0f};
+          float ca[4], cb[4];
+          if (normals) {
+            const float4 n0 = normals[r.ia], n1 = normals[r.ib];
+            na[0] = n0.x; na[1] = n0.y; na[2] = n0.z;
+            nb[0] = n1.x; nb[1] = n1.y; nb[2] = n1.z;
           }
+          if (colours) {
+            const uint32_t c0 = colours[r.ia], c1 = colours[r.ib];
+            for (int k = 0; k < 4; ++k) {
+              ca[k] = float((c0 >> (24 - 8 * k)) & 0xFFu);
+              cb[k] = float((c1 >> (24 - 8 * k)) & 0xFFu);
+            }
+          } else {
+            for (int k = 0; k < 4; ++k) ca[k] = cb[k] = 255.0f;
+          }
+          float* nd = triNrm + slot * 9 + 3 * v;
+          for (int k = 0; k < 3; ++k) nd[k] = __builtin_fmaf(nb[k] - na[k], r.t, na[k]);
+          uint32_t packed = 0;
+          for (int k = 0; k < 4; ++k) {
+            const float mv = __builtin_fmaf(cb[k] - ca[k], r.t, ca[k]);
+            const uint32_t u = !(mv > 0.0f) ? 0u : (mv >= 255.0f ? 255u : uint32_t(mv));
+            packed = (packed << 8) | u;
+          }
+          triCol[slot * 3 + v] = packed;
         }
       }
     }
@@ -581,20 +605,26 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
   return LF_OK;
 }
 
+// Cube indices are 32-bit in the kernels (with a block of headroom).
+static bool cubesFit(const Ctx* c) {
+  return uint64_t(c->S[0] - 1) * (c->S[1] - 1) * (c->S[2] - 1) < (uint64_t(1) << 31);
+}
+
 static CubeGrid cubeGrid(const Ctx* c) {
   CubeGrid g{};
   for (int i = 0; i < 3; ++i) {
     g.S[i] = c->S[i];
     g.M[i] = c->S[i] - 1u;
   }
-  g.cubes = uint64_t(g.M[0]) * g.M[1] * g.M[2];
+  g.cubes = uint32_t(uint64_t(g.M[0]) * g.M[1] * g.M[2]);
   return g;
 }
 
 int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
   if (!c->mcSampled) return fail(c, LF_ERR_STATE, "lf_mc_classify_count: no field (call lf_mc_sample)");
+  if (!cubesFit(c)) return fail(c, LF_ERR_ARG, "marching cubes: more than 2^31 cubes");
   const CubeGrid g = cubeGrid(c);
-  const uint64_t nbl = (g.cubes + kCubesPerBlock - 1) / kCubesPerBlock;
+  const uint64_t nbl = (uint64_t(g.cubes) + kCubesPerBlock - 1) / kCubesPerBlock;
   int rc = ensure(c, c->mcBlock, size_t(nbl) + 1, "mc block counts");
   if (rc) return rc;
   if ((rc = ensure(c, c->mcTotal, 4, "mc total"))) return rc;
@@ -615,7 +645,7 @@ int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
 int mcEmit(Ctx* c, const lf_mc_params& m) {
   if (!c->mcCounted) return fail(c, LF_ERR_STATE, "lf_mc_emit: no counts (call lf_mc_classify_count)");
   const CubeGrid g = cubeGrid(c);
-  const uint64_t nbl = (g.cubes + kCubesPerBlock - 1) / kCubesPerBlock;
+  const uint64_t nbl = (uint64_t(g.cubes) + kCubesPerBlock - 1) / kCubesPerBlock;
   int rc = ensure(c, c->triPos, c->triCap * 9, "triangle positions");
   if (rc) return rc;
   const int attr = (m.attributes != 0) ? 1 : 0;
@@ -638,7 +668,8 @@ int mcEmit(Ctx* c, const lf_mc_params& m) {
                 (const float*)c->field.p,
                 (const float4*)((attr && c->mcAttr) ? c->mcNormal.p : nullptr),
                 (const uint32_t*)((attr && c->mcAttr) ? c->mcColour.p : nullptr),
-                (const uint32_t*)c->mcBlock.p, c->triPos.p, attr ? c->triNrm.p : (float*)nullptr,
+                (const uint32_t*)c->mcBlock.p, (const uint32_t*)c->mcTotal.p, uint32_t(nbl),
+                c->triPos.p, attr ? c->triNrm.p : (float*)nullptr,
                 attr ? c->triCol.p : (uint32_t*)nullptr);
     if (rc) return rc;
   }
