@@ -204,7 +204,18 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
         // particles are one contiguous run of the staging, in visit order
         const uint32_t row = ((bz + dz) * hy + (by + dy)) * hx + bx;
         const uint32_t e = cellStart[row + 3];
-        for (uint32_t b = cellStart[row]; b < e; ++b)
+        uint32_t b = cellStart[row];
+        // four staged particles in flight, terms added in order
+        for (; b + 4 <= e; b += 4) {
+          float t[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            t[u] = densityTerm(c, make_float4(stX[b + u], stY[b + u], stZ[b + u], stM[b + u]),
+                               px, py, pz);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) rho += t[u];
+        }
+        for (; b < e; ++b)
           rho += densityTerm(c, make_float4(stX[b], stY[b], stZ[b], stM[b]), px, py, pz);
       }
   } else {
