@@ -16,9 +16,9 @@
  * Morton order (:874-892); queries report the fluid particles of the query
  * point's own cell (:770-791). Link with libfluid_amd/liblfamd.so.
  *
- * Errors throw std::runtime_error carrying lf_last_error(). Mesh colliders
- * (oclsph_collision.h) are not implemented: advance() throws when a collider
- * holds triangles rather than ignore it.
+ * Mesh colliders (oclsph_collision.h) run inside every delta pass: the
+ * triangles of all colliders, concatenated in order (oclsph.hpp:796-807).
+ * Errors throw std::runtime_error carrying lf_last_error().
  */
 #ifndef LIBFLUID_AMD_HIPSPH_HPP
 #define LIBFLUID_AMD_HIPSPH_HPP
@@ -70,9 +70,18 @@ class SphSolver final : public fluid::SphSolver<size_t, float> {
   fluid::Result<float> advance(const fluid::Config<float>& config,
                                std::vector<fluid::Particle<size_t, float>>& xs,
                                const std::vector<fluid::MeshCollider<float>>& colliders) override {
+    std::vector<lf_triangle> mesh;  // oclsph.hpp:796-807
     for (const auto& c : colliders)
-      if (!c.triangles.empty())
-        throw std::runtime_error("hip::SphSolver: mesh colliders are not supported");
+      for (const auto& t : c.triangles) {
+        lf_triangle q;
+        for (int k = 0; k < 3; ++k) {
+          q.v0[k] = t.v0[k];
+          q.v1[k] = t.v1[k];
+          q.v2[k] = t.v2[k];
+        }
+        mesh.push_back(q);
+      }
+    check(lf_set_colliders(ctx, mesh.size(), mesh.data()));
 
     // sources then drains (oclsph.hpp:678-706)
     const float spacing = h * config.scale / 2;

@@ -84,6 +84,12 @@ typedef struct lf_pbf_params {
  * scale 1, 4 iterations, no gravity, unit box, XSPH off). */
 void lf_pbf_params_default(lf_pbf_params* p, float h);
 
+/* A collider triangle in world units: geometry::Triangle<float>
+ * (geometry.hpp) as MeshCollider holds it (fluid.hpp:121-126). */
+typedef struct lf_triangle {
+  float v0[3], v1[3], v2[3];
+} lf_triangle;
+
 /* A gravity well (fluid.hpp:137-144); applied in the predict pass exactly as
  * oclsph.hpp:306-317 (radius 75 world units, force clamped to +-10). */
 typedef struct lf_well {
@@ -151,6 +157,15 @@ int lf_state_download(lf_ctx* ctx, size_t n, float* position, float* velocity,
                       uint32_t* order);
 int lf_state_size(const lf_ctx* ctx, size_t* n);
 int lf_set_wells(lf_ctx* ctx, size_t n, const lf_well* wells);
+/* Mesh colliders for the delta pass (collideTriangle2,
+ * oclsph_collision.h:25-58, called from sph_delta, oclsph_kernel.h:364-368):
+ * the triangles of all MeshColliders concatenated in order, as advance()
+ * builds them (oclsph.hpp:796-807). Every delta pass tests each particle's
+ * new position against every triangle, in order; at the first triangle whose
+ * plane is within 40 world units and whose plane projection lies inside it,
+ * the particle moves to the projection and its velocity is reflected about
+ * the plane. n = 0 removes them. Every scene of a batch sees the same mesh. */
+int lf_set_colliders(lf_ctx* ctx, size_t n, const lf_triangle* tris);
 
 /* ---- the PBF step, stage by stage --------------------------------------- */
 /* Predict (advectAndCopy, oclsph.hpp:289-325), Morton key

@@ -162,6 +162,11 @@ class Context:
             arr[i].force = w.force
         self._check(self._lib.lf_set_wells(self._h, len(wells), ctypes.cast(arr, ctypes.c_void_p)))
 
+    def set_colliders(self, triangles):
+        """Mesh colliders: float array (n, 3, 3) of triangle corners, world units."""
+        t = np.ascontiguousarray(np.asarray(triangles, np.float32).reshape(-1, 9))
+        self._check(self._lib.lf_set_colliders(self._h, t.shape[0], _ptr(t) if t.size else None))
+
     # -- stages ---------------------------------------------------------------
     def grid_build(self, p):
         self._check(self._lib.lf_grid_build(self._h, ctypes.byref(p)))

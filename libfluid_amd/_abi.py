@@ -170,6 +170,7 @@ SIGNATURES = {
     "lf_state_download": (_I, [_P, _SZ, _P, _P, _P, _P, _P, _P]),
     "lf_state_size": (_I, [_P, ctypes.POINTER(_SZ)]),
     "lf_set_wells": (_I, [_P, _SZ, _P]),
+    "lf_set_colliders": (_I, [_P, _SZ, _P]),
     "lf_grid_build": (_I, [_P, ctypes.POINTER(PbfParams)]),
     "lf_pbf_diffuse": (_I, [_P, ctypes.POINTER(PbfParams)]),
     "lf_pbf_lambda": (_I, [_P, ctypes.POINTER(PbfParams)]),

@@ -274,6 +274,53 @@ __device__ inline void predictPstar(const PbfConsts& c, float4 x, float4 v, uint
   pz = __builtin_fmaf(vz, c.dt, x.z / c.scale);
 }
 
+// ---- mesh colliders (oclsph_collision.h) --------------------------------------
+// Per-triangle values of collideTriangle2 / inTrig that do not depend on the
+// particle, folded once on the host in FP32 with the same operations as the
+// CPU restatement: nn = n / |n| with n = (b - a) x (c - a), dn = nn . a,
+// v0 = c - a, v1 = b - a, dot00, dot01, dot11, invDenom = 1 / (dot00 dot11 -
+// dot01^2); dot3(u, v) = fma(u.z, v.z, fma(u.y, v.y, u.x * v.x)).
+struct ColliderTri {
+  float a[3], nn[3], v0[3], v1[3];
+  float dn, dot00, dot01, dot11, invDenom;
+  float pad[3];
+};
+
+__host__ __device__ inline float dot3(const float* u, const float* v) {
+  return __builtin_fmaf(u[2], v[2], __builtin_fmaf(u[1], v[1], u[0] * v[0]));
+}
+
+// collideTriangle2 (oclsph_collision.h:25-58): the first triangle, in order,
+// whose plane lies within 40 world units of p (t = dn - nn.p) and whose
+// projection p0 = p + nn t falls inside it (inTrig, :7-22: u >= 0, v >= 0,
+// u + v < 1) moves p to p0 and reflects v about the plane (v - nn (2 v.nn)).
+__device__ inline bool collideTriangles(const ColliderTri* __restrict__ tris, uint32_t n,
+                                        float p[3], float v[3]) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const ColliderTri& T = tris[i];
+    const float t = T.dn - dot3(T.nn, p);
+    if (!(-40.0f <= t && t <= 40.0f)) continue;
+    float p0[3], v2[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      p0[k] = __builtin_fmaf(T.nn[k], t, p[k]);
+      v2[k] = p0[k] - T.a[k];
+    }
+    const float dot02 = dot3(T.v0, v2), dot12 = dot3(T.v1, v2);
+    const float u = (T.dot11 * dot02 - T.dot01 * dot12) * T.invDenom;
+    const float w = (T.dot00 * dot12 - T.dot01 * dot02) * T.invDenom;
+    if (!(u >= 0.0f && w >= 0.0f && u + w < 1.0f)) continue;
+    const float s = 2.0f * dot3(v, T.nn);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      p[k] = p0[k];
+      v[k] = __builtin_fmaf(-T.nn[k], s, v[k]);
+    }
+    return true;
+  }
+  return false;
+}
+
 // ---- context ----------------------------------------------------------------
 template <typename T> struct DevBuf {
   T* p = nullptr;
@@ -313,6 +360,8 @@ struct Ctx {
   DevBuf<uint32_t> sortHist;
   DevBuf<lf_well> wells;
   int nWells = 0;
+  DevBuf<ColliderTri> colliders;
+  uint32_t nColliders = 0;
   std::vector<lf_well> hostWells;
 
   bool gridValid = false;

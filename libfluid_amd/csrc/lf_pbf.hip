@@ -299,7 +299,7 @@ __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float d
   az = __builtin_fmaf(dz * sc, factor, az);
 }
 
-// sph_delta (oclsph_kernel.h:335-384) without colliders; Jacobi: reads PL,
+// sph_delta (oclsph_kernel.h:335-384) with the mesh colliders; Jacobi: reads PL,
 // writes Pout. deltaP is not stored (nothing reads it after the pass). Also
 // raises D and the per-cell displacement to the new positions' (one atomic
 // per wave, and per run of equal keys).
@@ -309,7 +309,9 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
                                               const uint8_t* __restrict__ T,
                                               const float4* __restrict__ PL,
                                               const float4* __restrict__ X, Lists L, int fresh,
-                                              float4* __restrict__ Pout) {
+                                              float4* __restrict__ Pout, float4* __restrict__ V,
+                                              const ColliderTri* __restrict__ mesh,
+                                              uint32_t meshN) {
   __shared__ float waveMax[BS / 64];
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
@@ -329,16 +331,25 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
             [&](uint32_t, float4 pb, float d2) {
               deltaTerm<FAST>(c, la, pb.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, ax, ay, az);
             });
-        // pos = (p* + dp) * scale, clamped to the bounds (:364-371), p* = pos/scale
+        // pos = (p* + dp) * scale, mesh colliders (:364-368), clamped to the
+        // bounds (:371), p* = pos/scale (:374)
         const float acc[3] = {ax, ay, az};
         const float p[3] = {pa.x, pa.y, pa.z};
-        float o[3];
+        float pos[3], o[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) pos[k] = (p[k] + acc[k]) * c.scale;
+        if (meshN) {
+          const float4 va = V[a];
+          float vel[3] = {va.x, va.y, va.z};
+          if (collideTriangles(mesh, meshN, pos, vel))
+            V[a] = make_float4(vel[0], vel[1], vel[2], va.w);  // velocity[a] = resp.velocity
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          float pos = (p[k] + acc[k]) * c.scale;
-          pos = (c.minB[k] < pos) ? pos : c.minB[k];
-          pos = (pos < c.maxB[k]) ? pos : c.maxB[k];
-          o[k] = pos / c.scale;
+          float q = pos[k];
+          q = (c.minB[k] < q) ? q : c.minB[k];
+          q = (q < c.maxB[k]) ? q : c.maxB[k];
+          o[k] = q / c.scale;
         }
         Pout[a] = make_float4(o[0], o[1], o[2], mass);
         const float4 p0 = L.P0[a];
@@ -564,7 +575,8 @@ int pbfDelta(Ctx* c, bool skipDone) {
                     dim3(decltype(bs)::value), 0, W, (const uint32_t*)c->localKey.p,
                     (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
                     (const float4*)c->PL.p, (const float4*)c->X[c->cur].p, listsOf(c),
-                    int(c->listFresh), c->P.p);
+                    int(c->listFresh), c->P.p, c->V[c->cur].p,
+                    (const ColliderTri*)c->colliders.p, c->nColliders);
     });
   });
   c->listFresh = false;  // p* now differs from the predicted positions

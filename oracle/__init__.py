@@ -36,6 +36,9 @@ def lib():
         L.oc_destroy.argtypes = [P]
         L.oc_load.argtypes = [P, ctypes.c_size_t, P, P, P, P, P]
         L.oc_set_wells.argtypes = [P, ctypes.c_size_t, P]
+        L.oc_set_colliders.argtypes = [P, ctypes.c_size_t, P]
+        L.oc_collide.restype = ctypes.c_int
+        L.oc_collide.argtypes = [P, P, P]
         L.oc_stage.restype = ctypes.c_double
         L.oc_stage.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.oc_size.restype = ctypes.c_size_t
@@ -122,6 +125,17 @@ class Pbf:
             arr[i].centre[:] = list(w.centre)
             arr[i].force = w.force
         self._L.oc_set_wells(self._h, len(wells), ctypes.cast(arr, ctypes.c_void_p))
+
+    def set_colliders(self, triangles):
+        t = np.ascontiguousarray(np.asarray(triangles, np.float32).reshape(-1, 9))
+        self._L.oc_set_colliders(self._h, t.shape[0], _p(t))
+
+    def collide(self, position, velocity):
+        """collideTriangle2 on one point: (hit, position, velocity)."""
+        p = np.array(position, np.float64)
+        v = np.array(velocity, np.float64)
+        hit = self._L.oc_collide(self._h, _p(p), _p(v))
+        return bool(hit), p, v
 
     def stage(self, s, repeat=1):
         """Run one stage; returns wall seconds."""

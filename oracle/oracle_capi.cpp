@@ -126,6 +126,27 @@ void oc_set_wells(void* hp, size_t n, const lf_well* w) {
     h->f->wells.assign(w, w + n);
 }
 
+void oc_set_colliders(void* hp, size_t n, const lf_triangle* t) {
+  auto* h = static_cast<Handle*>(hp);
+  if (h->fp64)
+    h->d->setColliders(n, t);
+  else
+    h->f->setColliders(n, t);
+}
+
+/* collideTriangle2 on one point (p, v in world units; updated in place). */
+int oc_collide(void* hp, double* p, double* v) {
+  auto* h = static_cast<Handle*>(hp);
+  if (h->fp64) return h->d->collide(p, v) ? 1 : 0;
+  float pf[3] = {float(p[0]), float(p[1]), float(p[2])}, vf[3] = {float(v[0]), float(v[1]), float(v[2])};
+  const int hit = h->f->collide(pf, vf) ? 1 : 0;
+  for (int k = 0; k < 3; ++k) {
+    p[k] = pf[k];
+    v[k] = vf[k];
+  }
+  return hit;
+}
+
 /* stage: 0 grid, 1 lambda, 2 delta, 3 finalise, 4 xsph, 5 diffuse, 6 commit,
  * 7 whole step. Returns the wall time in seconds. */
 double oc_stage(void* hp, int s, int repeat) {

@@ -179,6 +179,64 @@ template <typename F> class Pbf {
   std::vector<int32_t> type;
   std::vector<uint32_t> colour;
   std::vector<lf_well> wells;
+
+  /* Mesh colliders (oclsph_collision.h): per-triangle values of
+   * collideTriangle2 / inTrig folded once, dot3 = fma(u.z,v.z, fma(u.y,v.y,
+   * u.x*v.x)), the unit normal nn = n / |n| of n = (b-a) x (c-a). */
+  struct ColliderTri {
+    F a[3], nn[3], v0[3], v1[3], dn, dot00, dot01, dot11, invDenom;
+  };
+  std::vector<ColliderTri> mesh;
+  static F dot3(const F* u, const F* v) { return std::fma(u[2], v[2], std::fma(u[1], v[1], u[0] * v[0])); }
+  void setColliders(size_t nt, const lf_triangle* tris) {
+    mesh.resize(nt);
+    for (size_t i = 0; i < nt; ++i) {
+      ColliderTri& T = mesh[i];
+      F e1[3], e2[3], nv[3];
+      for (int k = 0; k < 3; ++k) {
+        T.a[k] = F(tris[i].v0[k]);
+        e1[k] = F(tris[i].v1[k]) - T.a[k];
+        e2[k] = F(tris[i].v2[k]) - T.a[k];
+      }
+      nv[0] = e1[1] * e2[2] - e1[2] * e2[1];
+      nv[1] = e1[2] * e2[0] - e1[0] * e2[2];
+      nv[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      const F len = std::sqrt(dot3(nv, nv));
+      for (int k = 0; k < 3; ++k) {
+        T.nn[k] = nv[k] / len;
+        T.v0[k] = e2[k];
+        T.v1[k] = e1[k];
+      }
+      T.dn = dot3(T.nn, T.a);
+      T.dot00 = dot3(T.v0, T.v0);
+      T.dot01 = dot3(T.v0, T.v1);
+      T.dot11 = dot3(T.v1, T.v1);
+      T.invDenom = F(1) / (T.dot00 * T.dot11 - T.dot01 * T.dot01);
+    }
+  }
+  /* collideTriangle2 (oclsph_collision.h:25-58) with inTrig (:7-22). */
+  bool collide(F p[3], F v[3]) const {
+    for (const ColliderTri& T : mesh) {
+      const F t = T.dn - dot3(T.nn, p);
+      if (!(F(-40) <= t && t <= F(40))) continue;
+      F p0[3], v2[3];
+      for (int k = 0; k < 3; ++k) {
+        p0[k] = std::fma(T.nn[k], t, p[k]);
+        v2[k] = p0[k] - T.a[k];
+      }
+      const F dot02 = dot3(T.v0, v2), dot12 = dot3(T.v1, v2);
+      const F u = (T.dot11 * dot02 - T.dot01 * dot12) * T.invDenom;
+      const F w = (T.dot00 * dot12 - T.dot01 * dot02) * T.invDenom;
+      if (!(u >= F(0) && w >= F(0) && u + w < F(1))) continue;
+      const F sv = F(2) * dot3(v, T.nn);
+      for (int k = 0; k < 3; ++k) {
+        p[k] = p0[k];
+        v[k] = std::fma(-T.nn[k], sv, v[k]);
+      }
+      return true;
+    }
+    return false;
+  }
   /* sorted state of the current step */
   std::vector<uint32_t> keyIn, keys, order, table;
   std::vector<F> ps, psNext, xs, vs, ms, lambda, rho;
@@ -342,7 +400,7 @@ template <typename F> class Pbf {
     }
   }
 
-  /* sph_delta (oclsph_kernel.h:335-384) without colliders, Jacobi. */
+  /* sph_delta (oclsph_kernel.h:335-384) with the mesh colliders, Jacobi. */
   void deltaPass() {
     psNext.resize(4 * n);
 #pragma omp parallel for schedule(dynamic, 256)
@@ -367,10 +425,12 @@ template <typename F> class Pbf {
         acc[1] = std::fma(dy * s, factor, acc[1]);
         acc[2] = std::fma(dz * s, factor, acc[2]);
       });
+      F pos[3];
+      for (int k = 0; k < 3; ++k) pos[k] = (pa[k] + acc[k]) * c.scale;
+      if (!mesh.empty()) collide(pos, &vs[3 * a]); /* :364-368, velocity[a] = resp.velocity */
       for (int k = 0; k < 3; ++k) {
-        F pos = (pa[k] + acc[k]) * c.scale;
-        pos = std::min(c.maxB[k], std::max(c.minB[k], pos)); /* :371 */
-        psNext[4 * a + k] = pos / c.scale;
+        const F q = std::min(c.maxB[k], std::max(c.minB[k], pos[k])); /* :371 */
+        psNext[4 * a + k] = q / c.scale;
       }
       psNext[4 * a + 3] = pa[3];
     }

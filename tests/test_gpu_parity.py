@@ -394,3 +394,35 @@ def test_mesh_arbitrary_field(ctx):
     otris = oracle.mc_emit(field.astype(np.float64), 0.0, (0, 0, 0), (float(step),) * 3, 2.0)
     assert len(tris) == len(otris) > 1000
     _eq(tris["v"], otris["v"], "sphere mesh")
+
+
+def _ramp_mesh():
+    """Two triangles forming a tilted ramp across the 8k box's floor region,
+    plus one far away that never touches (plane beyond 40 units)."""
+    a, b, c, d = (0.0, 0.05, 0.0), (2.0, 0.35, 0.0), (2.0, 0.35, 1.0), (0.0, 0.05, 1.0)
+    far = ((100.0, 100.0, 100.0), (101.0, 100.0, 100.0), (100.0, 101.0, 100.0))
+    return np.array([(a, b, c), (a, c, d), far], np.float32)
+
+
+@pytest.mark.parametrize("steps", [1, 5])
+def test_mesh_collider_bit_exact(ctx, steps):
+    """collideTriangle2 inside every delta pass (oclsph_collision.h:25-58,
+    oclsph_kernel.h:364-368): positions and the reflected velocities."""
+    sc = scenes.dam_break_8k()
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
+    o = _pair(ctx, sc, p)
+    mesh = _ramp_mesh()
+    ctx.set_colliders(mesh)
+    o.set_colliders(mesh)
+    try:
+        ctx.step(p, steps)
+        o.step(steps)
+        g = ctx.download()
+        _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position")
+        _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity")
+    finally:
+        ctx.set_colliders(np.zeros((0, 3, 3), np.float32))
+    # the ramp moved particles: the same steps without it differ
+    o2 = _pair(ctx, sc, p)
+    ctx.step(p, steps)
+    assert not np.array_equal(ctx.download()["position"], g["position"])

@@ -362,3 +362,23 @@ def test_mc_sphere_area_and_closure():
     edges = np.sort(np.concatenate([inv[:, [0, 1]], inv[:, [1, 2]], inv[:, [2, 0]]]), axis=1)
     _, counts = np.unique(edges, axis=0, return_counts=True)
     assert np.all(counts == 2)
+
+
+def test_collider_closed_form():
+    """collideTriangle2 (oclsph_collision.h:25-58): a point above the triangle
+    (0,0,0),(1,0,0),(0,0,1) projects onto its plane and the velocity reflects;
+    a point whose projection falls outside, or whose plane is farther than 40
+    units, is untouched; the first hit in order wins."""
+    sc = scenes.small_block(n_side=4)
+    for fp64 in (False, True):
+        o = oracle.Pbf(sc.params(), fp64=fp64)
+        o.set_colliders(np.array([[(0, 0, 0), (1, 0, 0), (0, 0, 1)],
+                                  [(0, 0.25, 0), (1, 0.25, 0), (0, 0.25, 1)]], np.float32))
+        hit, p, v = o.collide((0.2, 0.5, 0.2), (0.3, -1.0, 0.1))
+        # the second triangle's plane (y = 0.25) comes second: the first wins
+        assert hit and np.allclose(p, (0.2, 0.0, 0.2), atol=1e-7)
+        assert np.allclose(v, (0.3, 1.0, 0.1), atol=1e-7)  # reflected about y
+        hit, p, v = o.collide((0.8, 0.5, 0.8), (0.0, -1.0, 0.0))  # u + v > 1: outside both
+        assert not hit and np.allclose(p, (0.8, 0.5, 0.8)) and np.allclose(v, (0, -1, 0))
+        hit, _, _ = o.collide((0.2, 41.0, 0.2), (0, 0, 0))  # plane 41 units away
+        assert not hit
