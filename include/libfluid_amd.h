@@ -97,6 +97,26 @@ typedef struct lf_well {
   float force;
 } lf_well;
 
+/* A particle source (fluid.hpp:146-159): every step adds a floor(sqrt(rate))
+ * x ceil(sqrt(rate)) lattice of fluid particles, spacing h*scale/2, centred
+ * on `centre` in x and z, mass 1, with the given velocity and colour
+ * (oclsph.hpp:678-691). */
+typedef struct lf_source {
+  float centre[3];
+  float velocity[3];
+  uint32_t rate;
+  uint32_t colour;
+} lf_source;
+
+/* A drain (fluid.hpp:161-169): every step erases the fluid particles whose
+ * distance to `centre` is below `width` (oclsph.hpp:693-706; depth unused, as
+ * in the reference). */
+typedef struct lf_drain {
+  float centre[3];
+  float width;
+  float depth;
+} lf_drain;
+
 /* Marching-cubes parameters. Replaces ClMcConfig (oclsph_type.h:39-44). */
 #define LF_MC_FIELD_REFERENCE 0 /* mc_lattice metaball field, lattice from the
                                    grid extent (oclsph_kernel.h:421-530,
@@ -166,6 +186,21 @@ int lf_set_wells(lf_ctx* ctx, size_t n, const lf_well* wells);
  * the particle moves to the projection and its velocity is reflected about
  * the plane. n = 0 removes them. Every scene of a batch sees the same mesh. */
 int lf_set_colliders(lf_ctx* ctx, size_t n, const lf_triangle* tris);
+/* Device-resident sources, drains and spatial queries for lf_pbf_step (the
+ * host pre-pass of advance(), oclsph.hpp:678-706 and :770-791, kept on the
+ * device). Each step first appends the sources' particles at the end of the
+ * state (upload indices continue after the largest one so far), then erases
+ * the drained fluid particles keeping the order of the rest (one 4-byte
+ * readback of the new count), then steps. Queries are answered right after
+ * the step's grid build, from the colours the step started with, as the
+ * reference does: per query point (world units) the fluid particles of its
+ * own cell, their count and mean ARGB colour; lf_query_results copies the
+ * last step's answers out (count 0 = no answer). Not for batches; a context
+ * with sources or drains cannot capture a graph. n = 0 removes them. */
+int lf_set_sources(lf_ctx* ctx, size_t n, const lf_source* sources);
+int lf_set_drains(lf_ctx* ctx, size_t n, const lf_drain* drains);
+int lf_set_queries(lf_ctx* ctx, size_t n, const float* points);
+int lf_query_results(lf_ctx* ctx, size_t n, uint32_t* counts, uint32_t* colours);
 
 /* ---- the PBF step, stage by stage --------------------------------------- */
 /* Predict (advectAndCopy, oclsph.hpp:289-325), Morton key

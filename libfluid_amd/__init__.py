@@ -167,6 +167,40 @@ class Context:
         t = np.ascontiguousarray(np.asarray(triangles, np.float32).reshape(-1, 9))
         self._check(self._lib.lf_set_colliders(self._h, t.shape[0], _ptr(t) if t.size else None))
 
+    def set_sources(self, sources):
+        """Sources appended on the device at the start of every lf_pbf_step:
+        each an object with centre, velocity, rate and colour."""
+        arr = (_abi.Source * max(1, len(sources)))()
+        for i, src in enumerate(sources):
+            arr[i].centre[:] = [float(v) for v in src.centre]
+            arr[i].velocity[:] = [float(v) for v in src.velocity]
+            arr[i].rate = int(src.rate)
+            arr[i].colour = int(src.colour) & 0xFFFFFFFF
+        self._check(self._lib.lf_set_sources(self._h, len(sources), arr if sources else None))
+
+    def set_drains(self, drains):
+        """Drains applied on the device after the sources of every lf_pbf_step."""
+        arr = (_abi.Drain * max(1, len(drains)))()
+        for i, d in enumerate(drains):
+            arr[i].centre[:] = [float(v) for v in d.centre]
+            arr[i].width = float(d.width)
+            arr[i].depth = float(d.depth)
+        self._check(self._lib.lf_set_drains(self._h, len(drains), arr if drains else None))
+
+    def set_queries(self, points):
+        """Query points (world units), answered by every lf_pbf_step after its grid build."""
+        pts = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
+        self._n_queries = pts.shape[0]
+        self._check(self._lib.lf_set_queries(self._h, pts.shape[0], _ptr(pts) if pts.size else None))
+
+    def query_results(self):
+        """(counts, colours) of the last step's answers to the query points."""
+        n = getattr(self, "_n_queries", 0)
+        counts = np.zeros(n, np.uint32)
+        colours = np.zeros(n, np.uint32)
+        self._check(self._lib.lf_query_results(self._h, n, _ptr(counts), _ptr(colours)))
+        return counts, colours
+
     # -- stages ---------------------------------------------------------------
     def grid_build(self, p):
         self._check(self._lib.lf_grid_build(self._h, ctypes.byref(p)))
@@ -188,6 +222,9 @@ class Context:
 
     def step(self, p, nsteps=1):
         self._check(self._lib.lf_pbf_step(self._h, ctypes.byref(p), nsteps))
+        size = ctypes.c_size_t()
+        self._check(self._lib.lf_state_size(self._h, ctypes.byref(size)))
+        self.n = size.value  # sources and drains change the particle count
 
     def sync(self):
         self._check(self._lib.lf_sync(self._h))
@@ -544,9 +581,9 @@ def _apply_sources_drains(config, xs, h):
     """Sources then drains, as oclsph.hpp:678-706 (host side of advance)."""
     xs = np.asarray(xs, dtype=particle_dtype())
     added = []
-    spacing = np.float32(h * config.scale / 2.0)
+    spacing = np.float32(h) * np.float32(config.scale) / np.float32(2)
     for src in config.sources:
-        size = math.sqrt(src.rate)
+        size = np.float32(math.sqrt(src.rate))
         width, depth = int(math.floor(size)), int(math.ceil(size))
         off = np.asarray(src.centre, np.float32) - (
             np.array([width, 0, depth], np.float32) / np.float32(2) * spacing)
@@ -565,7 +602,8 @@ def _apply_sources_drains(config, xs, h):
     if config.drains and len(xs):
         keep = np.ones(len(xs), bool)
         for d in config.drains:
-            dist = np.linalg.norm(xs["position"] - np.asarray(d.centre, np.float32), axis=1)
+            r = np.asarray(d.centre, np.float32) - xs["position"]
+            dist = np.sqrt((r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]) + r[:, 2] * r[:, 2])
             keep &= ~((xs["type"] != LF_OBSTACLE) & (dist < d.width))
         xs = xs[keep]
     return xs

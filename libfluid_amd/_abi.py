@@ -108,6 +108,18 @@ class MeshTriangle(ctypes.Structure):
     ]
 
 
+class Source(ctypes.Structure):
+    """lf_source: a particle source (fluid::Source<float>, fluid.hpp:147-159), without the tag."""
+    _fields_ = [("centre", ctypes.c_float * 3), ("velocity", ctypes.c_float * 3),
+                ("rate", ctypes.c_uint32), ("colour", ctypes.c_uint32)]
+
+
+class Drain(ctypes.Structure):
+    """lf_drain: a spherical drain (fluid::Drain<float>, fluid.hpp:162-169)."""
+    _fields_ = [("centre", ctypes.c_float * 3), ("width", ctypes.c_float),
+                ("depth", ctypes.c_float)]
+
+
 class Diag(ctypes.Structure):
     """lf_diag: FP64 per-scene conserved quantities and density error."""
     _fields_ = [
@@ -171,6 +183,10 @@ SIGNATURES = {
     "lf_state_size": (_I, [_P, ctypes.POINTER(_SZ)]),
     "lf_set_wells": (_I, [_P, _SZ, _P]),
     "lf_set_colliders": (_I, [_P, _SZ, _P]),
+    "lf_set_sources": (_I, [_P, _SZ, _P]),
+    "lf_set_drains": (_I, [_P, _SZ, _P]),
+    "lf_set_queries": (_I, [_P, _SZ, _P]),
+    "lf_query_results": (_I, [_P, _SZ, _P, _P]),
     "lf_grid_build": (_I, [_P, ctypes.POINTER(PbfParams)]),
     "lf_pbf_diffuse": (_I, [_P, ctypes.POINTER(PbfParams)]),
     "lf_pbf_lambda": (_I, [_P, ctypes.POINTER(PbfParams)]),

@@ -251,9 +251,57 @@ static int batchOnly(Ctx* c, const char* who) {
               std::string(who) + ": the context holds a batch (use lf_batch_step / lf_batch_select)");
 }
 
+// Sources of oclsph.hpp:678-691, appended at the end of the state: a
+// floor(sqrt(rate)) x ceil(sqrt(rate)) lattice per source, spacing h*scale/2,
+// offset = centre - (width, 0, depth)/2 * spacing, in the reference's float
+// operation order; mass 1, fluid.
+static int appendSources(Ctx* c) {
+  if (c->sources.empty()) return LF_OK;
+  const float spacing = c->params.h * c->params.scale / 2.0f;
+  std::vector<float4> X, V;
+  for (const lf_source& s : c->sources) {
+    const float size = float(std::sqrt(double(s.rate)));  // sqrt of an integer rate is taken in double
+    const int width = int(std::floor(size)), depth = int(std::ceil(size));
+    const float off[3] = {s.centre[0] - (float(width) / 2.0f) * spacing,
+                          s.centre[1] - (0.0f / 2.0f) * spacing,
+                          s.centre[2] - (float(depth) / 2.0f) * spacing};
+    float cw;
+    std::memcpy(&cw, &s.colour, 4);
+    for (int x = 0; x < width; ++x)
+      for (int z = 0; z < depth; ++z) {
+        X.push_back(make_float4(off[0] + float(x) * spacing, off[1] + 0.0f * spacing,
+                                off[2] + float(z) * spacing, 1.0f));
+        V.push_back(make_float4(s.velocity[0], s.velocity[1], s.velocity[2], cw));
+      }
+  }
+  const size_t k = X.size(), n = c->n;
+  if (k == 0) return LF_OK;
+  if (n + k >= (size_t(1) << 31)) return fail(c, LF_ERR_ARG, "sources: too many particles");
+  const int cur = c->cur;
+  int rc;
+  if ((rc = ensureKeep(c, c->X[cur], n + k, n, "positions"))) return rc;
+  if ((rc = ensureKeep(c, c->V[cur], n + k, n, "velocities"))) return rc;
+  if ((rc = ensureKeep(c, c->T[cur], n + k, n, "types"))) return rc;
+  if ((rc = ensureKeep(c, c->ID[cur], n + k, n, "ids"))) return rc;
+  std::vector<uint8_t> T(k, uint8_t(LF_FLUID));
+  std::vector<uint32_t> ID(k);
+  for (size_t i = 0; i < k; ++i) ID[i] = c->nextId++;
+  LF_HIP(c, hipMemcpyAsync(c->X[cur].p + n, X.data(), k * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+  LF_HIP(c, hipMemcpyAsync(c->V[cur].p + n, V.data(), k * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+  LF_HIP(c, hipMemcpyAsync(c->T[cur].p + n, T.data(), k, hipMemcpyHostToDevice, c->stream));
+  LF_HIP(c, hipMemcpyAsync(c->ID[cur].p + n, ID.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  LF_HIP(c, hipStreamSynchronize(c->stream));
+  c->n = n + k;
+  c->sceneBegin = {0u, uint32_t(c->n)};
+  return LF_OK;
+}
+
 static int pbfStepOnce(Ctx* c) {
   int rc;
+  if ((rc = appendSources(c))) return rc;
+  if ((rc = applyDrains(c))) return rc;
   if ((rc = gridBuild(c))) return rc;
+  if (c->nQueries && (rc = answerQueries(c))) return rc;
   if ((rc = pbfBuildLists(c))) return rc;
   if (c->consts.flags & LF_FLAG_DIFFUSE)
     if ((rc = pbfDiffuse(c))) return rc;
@@ -346,6 +394,10 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->sortHist);
   lf::release(c->wells);
   lf::release(c->colliders);
+  lf::release(c->drains);
+  lf::release(c->queryPts);
+  lf::release(c->queryOut);
+  lf::release(c->compactBlock);
   lf::release(c->field);
   lf::release(c->mcNormal);
   lf::release(c->mcColour);
@@ -413,6 +465,8 @@ int lf_state_upload(lf_ctx* c, size_t n, const float* pos, const float* vel, con
     LF_HIP(c, hipStreamSynchronize(c->stream));
   }
   c->n = n;
+  c->nextId = uint32_t(n);
+  c->queriesAnswered = false;
   c->batch = false;
   c->sceneParams.clear();
   c->sceneBegin = {0u, uint32_t(n)};
@@ -475,6 +529,66 @@ int lf_set_wells(lf_ctx* c, size_t n, const lf_well* wells) {
     LF_HIP(c, hipStreamSynchronize(c->stream));
   }
   c->nWells = int(n);
+  return LF_OK;
+}
+
+int lf_set_sources(lf_ctx* c, size_t n, const lf_source* src) {
+  if (!c) return LF_ERR_ARG;
+  if (n > 0 && !src) return fail(c, LF_ERR_ARG, "lf_set_sources: null sources");
+  if (n > 0 && c->batch) return fail(c, LF_ERR_STATE, "lf_set_sources: not for batches");
+  c->sources.assign(src, src + n);
+  lf::dropGraph(c);
+  return LF_OK;
+}
+
+int lf_set_drains(lf_ctx* c, size_t n, const lf_drain* d) {
+  if (!c) return LF_ERR_ARG;
+  if (n > 0 && !d) return fail(c, LF_ERR_ARG, "lf_set_drains: null drains");
+  if (n > 0 && c->batch) return fail(c, LF_ERR_STATE, "lf_set_drains: not for batches");
+  LF_HIP(c, hipSetDevice(c->device));
+  if (n > 0) {
+    int rc = lf::ensure(c, c->drains, n, "drains");
+    if (rc) return rc;
+    LF_HIP(c, hipMemcpyAsync(c->drains.p, d, n * sizeof(lf_drain), hipMemcpyHostToDevice, c->stream));
+    LF_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  c->hostDrains.assign(d, d + n);
+  lf::dropGraph(c);
+  return LF_OK;
+}
+
+int lf_set_queries(lf_ctx* c, size_t n, const float* pts) {
+  if (!c) return LF_ERR_ARG;
+  if (n > 0 && !pts) return fail(c, LF_ERR_ARG, "lf_set_queries: null points");
+  if (n > 0 && c->batch) return fail(c, LF_ERR_STATE, "lf_set_queries: not for batches");
+  if (n >= (size_t(1) << 30)) return fail(c, LF_ERR_ARG, "lf_set_queries: too many points");
+  LF_HIP(c, hipSetDevice(c->device));
+  if (n > 0) {
+    int rc = lf::ensure(c, c->queryPts, 3 * n, "query points");
+    if (rc) return rc;
+    LF_HIP(c, hipMemcpyAsync(c->queryPts.p, pts, 3 * n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    LF_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  lf::dropGraph(c);
+  c->nQueries = uint32_t(n);
+  c->queriesAnswered = false;
+  return LF_OK;
+}
+
+int lf_query_results(lf_ctx* c, size_t n, uint32_t* counts, uint32_t* colours) {
+  if (!c) return LF_ERR_ARG;
+  if (n != c->nQueries) return fail(c, LF_ERR_ARG, "lf_query_results: n differs from the query count");
+  if (!c->queriesAnswered) return fail(c, LF_ERR_STATE, "lf_query_results: no step since lf_set_queries");
+  LF_HIP(c, hipSetDevice(c->device));
+  std::vector<uint32_t> out(2 * n);
+  if (n) {
+    LF_HIP(c, hipMemcpyAsync(out.data(), c->queryOut.p, 2 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    LF_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  for (size_t i = 0; i < n; ++i) {
+    if (counts) counts[i] = out[2 * i];
+    if (colours) colours[i] = out[2 * i + 1];
+  }
   return LF_OK;
 }
 
@@ -879,6 +993,8 @@ int lf_batch_upload(lf_ctx* c, uint32_t nScenes, const lf_pbf_params* params,
     const int rc = lf::foldConsts(b, c->skinFrac, k, err);
     if (rc) return fail(c, rc, "scene " + std::to_string(s) + ": " + err);
   }
+  if (!c->sources.empty() || !c->hostDrains.empty() || c->nQueries)
+    return fail(c, LF_ERR_STATE, "lf_batch_upload: sources, drains and queries are single-scene only");
   int rc = lf_state_upload(c, n, pos, vel, mass, type, colour);
   if (rc) return rc;
   c->batch = true;
@@ -1040,6 +1156,8 @@ int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
     return fail(c, LF_ERR_ARG, "lf_graph_capture: nsteps must be even and > 0");
   if (!c->stepDone || c->n == 0)
     return fail(c, LF_ERR_STATE, "lf_graph_capture: run one eager step with these sizes first");
+  if (!c->sources.empty() || !c->hostDrains.empty())
+    return fail(c, LF_ERR_STATE, "lf_graph_capture: sources and drains change the particle count");
   LF_HIP(c, hipSetDevice(c->device));
   lf::dropGraph(c);
   int rc = lf::prepareScenes(c);  // uploads now, not inside the capture

@@ -154,7 +154,195 @@ __global__ __launch_bounds__(kThreads) void k_copy_probe(const float4* __restric
   for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// ---- drains: stable erase of drained fluid particles --------------------------
+constexpr uint32_t kCompactPerThread = 4;
+constexpr uint32_t kCompactBlock = kThreads * kCompactPerThread;
+
+// oclsph.hpp:693-706: a fluid particle is erased when glm::distance(centre,
+// position) < width for some drain (world units; dot product left to right).
+__device__ inline bool drained(const float4 x, uint8_t t, const lf_drain* __restrict__ d, int nd) {
+  if (t == LF_OBSTACLE) return false;
+  for (int i = 0; i < nd; ++i) {
+    const float dx = d[i].centre[0] - x.x, dy = d[i].centre[1] - x.y, dz = d[i].centre[2] - x.z;
+    if (__builtin_sqrtf((dx * dx + dy * dy) + dz * dz) < d[i].width) return true;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(kThreads) void k_drain_count(uint32_t n, const float4* __restrict__ X,
+                                                          const uint8_t* __restrict__ T,
+                                                          const lf_drain* __restrict__ d, int nd,
+                                                          uint32_t* __restrict__ blockCount) {
+  __shared__ uint32_t sm[kThreads / 64];
+  uint32_t keep = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kCompactPerThread; ++j) {
+    const uint32_t i = blockIdx.x * kCompactBlock + j * kThreads + threadIdx.x;
+    if (i < n && !drained(X[i], T[i], d, nd)) ++keep;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) keep += __shfl_xor(keep, o, 64);
+  if (__lane_id() == 0) sm[threadIdx.x >> 6] = keep;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kThreads / 64; ++w) t += sm[w];
+    blockCount[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of the block counts in place (one workgroup of 1024), total
+// into counts[nb].
+__global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ counts, uint32_t nb) {
+  __shared__ uint32_t sm[16];
+  const uint32_t per = (nb + 1023) / 1024;
+  const uint32_t lo = threadIdx.x * per, hi = min(lo + per, nb);
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; ++i) s += counts[i];
+  uint32_t x = s;
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sm[w] = x;
+  __syncthreads();
+  uint32_t wb = 0, tot = 0;
+  for (int i = 0; i < 16; ++i) {
+    if (i < w) wb += sm[i];
+    tot += sm[i];
+  }
+  uint32_t run = wb + x - s;
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t t = counts[i];
+    counts[i] = run;
+    run += t;
+  }
+  if (threadIdx.x == 0) counts[nb] = tot;
+}
+
+// Kept particles to their scanned slots, in order (ballot prefix per round).
+__global__ __launch_bounds__(kThreads) void k_drain_scatter(
+    uint32_t n, const float4* __restrict__ Xin, const float4* __restrict__ Vin,
+    const uint8_t* __restrict__ Tin, const uint32_t* __restrict__ IDin,
+    const lf_drain* __restrict__ d, int nd, const uint32_t* __restrict__ blockOffset,
+    float4* __restrict__ Xout, float4* __restrict__ Vout, uint8_t* __restrict__ Tout,
+    uint32_t* __restrict__ IDout) {
+  __shared__ uint32_t sm[kThreads / 64];
+  uint32_t run = blockOffset[blockIdx.x];
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+  for (uint32_t j = 0; j < kCompactPerThread; ++j) {
+    const uint32_t i = blockIdx.x * kCompactBlock + j * kThreads + threadIdx.x;
+    float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint8_t t = 0;
+    bool keep = false;
+    if (i < n) {
+      x = Xin[i];
+      t = Tin[i];
+      keep = !drained(x, t, d, nd);
+    }
+    const uint64_t bb = __ballot(keep);
+    if (lane == 0) sm[w] = uint32_t(__popcll(bb));
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < kThreads / 64; ++k) {
+      if (k < w) before += sm[k];
+      total += sm[k];
+    }
+    __syncthreads();
+    if (keep) {
+      const uint32_t o = run + before + uint32_t(__popcll(bb & ((uint64_t(1) << lane) - 1u)));
+      Xout[o] = x;
+      Vout[o] = Vin[i];
+      Tout[o] = t;
+      IDout[o] = IDin[i];
+    }
+    run += total;
+  }
+}
+
+// ---- spatial queries (oclsph.hpp:770-791) ---------------------------------------
+// Per query point (world units): the fluid particles of its own cell, their
+// count and the mean of their ARGB bytes packed back (truncated), from the
+// colours the step started with.
+__global__ __launch_bounds__(kThreads) void k_queries(uint32_t nq, const float* __restrict__ pts,
+                                                      PbfConsts c,
+                                                      const uint32_t* __restrict__ table,
+                                                      const uint8_t* __restrict__ T,
+                                                      const float4* __restrict__ V,
+                                                      uint32_t* __restrict__ out) {
+  const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
+  if (q >= nq) return;
+  const uint32_t key = mortonEncode(cellCoord((pts[3 * q] / c.scale - c.minExt[0]) / c.h),
+                                    cellCoord((pts[3 * q + 1] / c.scale - c.minExt[1]) / c.h),
+                                    cellCoord((pts[3 * q + 2] / c.scale - c.minExt[2]) / c.h));
+  uint32_t N = 0, packed = 0;
+  if (key + 1u < c.G) {
+    float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t e = table[key + 1u];
+    for (uint32_t a = table[key]; a < e; ++a) {
+      if (T[a] != LF_FLUID) continue;
+      const uint32_t col = __float_as_uint(V[a].w);
+      ++N;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) avg[k] += float((col >> (24 - 8 * k)) & 0xFFu);
+    }
+    if (N)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) packed = (packed << 8) | (uint32_t(uint8_t(avg[k] / float(N))) & 0xFFu);
+  }
+  out[2 * q] = N;
+  out[2 * q + 1] = packed;
+}
+
 }  // namespace
+
+int applyDrains(Ctx* c) {
+  const size_t n = c->n;
+  if (n == 0 || c->hostDrains.empty()) return LF_OK;
+  const uint32_t nb = uint32_t((n + kCompactBlock - 1) / kCompactBlock);
+  int rc;
+  const int nxt = 1 - c->cur;
+  if ((rc = ensure(c, c->compactBlock, size_t(nb) + 1, "drain counts"))) return rc;
+  if ((rc = ensure(c, c->X[nxt], n, "positions"))) return rc;
+  if ((rc = ensure(c, c->V[nxt], n, "velocities"))) return rc;
+  if ((rc = ensure(c, c->T[nxt], n, "types"))) return rc;
+  if ((rc = ensure(c, c->ID[nxt], n, "ids"))) return rc;
+  const int nd = int(c->hostDrains.size());
+  rc = launch(c, "drain_count", k_drain_count, dim3(nb), dim3(kThreads), 0, uint32_t(n),
+              (const float4*)c->X[c->cur].p, (const uint8_t*)c->T[c->cur].p,
+              (const lf_drain*)c->drains.p, nd, c->compactBlock.p);
+  if (rc) return rc;
+  rc = launch(c, "drain_scan", k_scan_counts, dim3(1), dim3(1024), 0, c->compactBlock.p, nb);
+  if (rc) return rc;
+  rc = launch(c, "drain_scatter", k_drain_scatter, dim3(nb), dim3(kThreads), 0, uint32_t(n),
+              (const float4*)c->X[c->cur].p, (const float4*)c->V[c->cur].p,
+              (const uint8_t*)c->T[c->cur].p, (const uint32_t*)c->ID[c->cur].p,
+              (const lf_drain*)c->drains.p, nd, (const uint32_t*)c->compactBlock.p, c->X[nxt].p,
+              c->V[nxt].p, c->T[nxt].p, c->ID[nxt].p);
+  if (rc) return rc;
+  uint32_t kept = 0;
+  LF_HIP(c, hipMemcpyAsync(&kept, c->compactBlock.p + nb, 4, hipMemcpyDeviceToHost, c->stream));
+  LF_HIP(c, hipStreamSynchronize(c->stream));
+  c->cur = nxt;
+  c->n = kept;
+  c->sceneBegin = {0u, kept};
+  return LF_OK;
+}
+
+int answerQueries(Ctx* c) {
+  if (c->nQueries == 0) return LF_OK;
+  int rc = ensure(c, c->queryOut, 2 * size_t(c->nQueries), "query results");
+  if (rc) return rc;
+  rc = launch(c, "queries", k_queries, dim3(blocksFor(c->nQueries, kThreads)), dim3(kThreads), 0,
+              c->nQueries, (const float*)c->queryPts.p, c->consts, (const uint32_t*)c->table.p,
+              (const uint8_t*)c->T[c->cur].p, (const float4*)c->V[c->cur].p, c->queryOut.p);
+  if (rc) return rc;
+  c->queriesAnswered = true;
+  return LF_OK;
+}
 
 int copyProbe(Ctx* c, size_t bytes, int reps, float* ms) {
   const size_t n = bytes / 16;

@@ -362,6 +362,15 @@ struct Ctx {
   int nWells = 0;
   DevBuf<ColliderTri> colliders;
   uint32_t nColliders = 0;
+  std::vector<lf_source> sources;
+  std::vector<lf_drain> hostDrains;
+  DevBuf<lf_drain> drains;
+  DevBuf<float> queryPts;       // 3 per query, world units
+  DevBuf<uint32_t> queryOut;    // per query: count, packed colour
+  uint32_t nQueries = 0;
+  bool queriesAnswered = false;
+  uint32_t nextId = 0;          // upload index of the next appended particle
+  DevBuf<uint32_t> compactBlock;  // per-block counts -> offsets of the drain compaction
   std::vector<lf_well> hostWells;
 
   bool gridValid = false;
@@ -457,6 +466,31 @@ template <typename T> int ensure(Ctx* c, DevBuf<T>& b, size_t count, const char*
   b.cap = want;
   return LF_OK;
 }
+// Grows b to hold count elements keeping its first `keep` elements.
+template <typename T>
+int ensureKeep(Ctx* c, DevBuf<T>& b, size_t count, size_t keep, const char* what) {
+  if (count <= b.cap) return LF_OK;
+  if (c->capturing)
+    return fail(c, LF_ERR_STATE, std::string(what) + ": buffer growth during graph capture");
+  dropGraph(c);
+  const size_t want = count + count / 4 + 256;
+  T* p = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+  if (e != hipSuccess) return hipFail(c, e, what);
+  if (b.p && keep) {
+    e = hipMemcpyAsync(p, b.p, keep * sizeof(T), hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return hipFail(c, e, what);
+    }
+  }
+  if (b.p) (void)hipFree(b.p);
+  b.p = p;
+  b.cap = want;
+  return LF_OK;
+}
+
 template <typename T> void release(DevBuf<T>& b) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
@@ -521,6 +555,8 @@ int mcSample(Ctx* c, const lf_mc_params& m);   // lf_mc.hip
 int mcClassifyCount(Ctx* c, const lf_mc_params& m);
 int mcEmit(Ctx* c, const lf_mc_params& m);
 int copyProbe(Ctx* c, size_t bytes, int reps, float* ms);  // lf_grid.hip
+int applyDrains(Ctx* c);                      // lf_grid.hip: stable erase, updates n
+int answerQueries(Ctx* c);                    // lf_grid.hip: after the grid build
 // lf_diag.hip: FP64 per-scene reductions. densityCheck runs after lambda pass
 // `pass` (convergence); diagState reduces the current state into the last slot.
 constexpr int kSumWidth = 8;   // count, mass, momentum x3, sum |e|, max |e|, unused

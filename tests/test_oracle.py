@@ -382,3 +382,44 @@ def test_collider_closed_form():
         assert not hit and np.allclose(p, (0.8, 0.5, 0.8)) and np.allclose(v, (0, -1, 0))
         hit, _, _ = o.collide((0.2, 41.0, 0.2), (0, 0, 0))  # plane 41 units away
         assert not hit
+
+
+def test_host_sources_and_drains_closed_form():
+    """Sources (oclsph.hpp:678-691): floor(sqrt(rate)) x ceil(sqrt(rate))
+    particles on an (x, z) lattice of spacing h*scale/2 centred on the source,
+    appended after the existing particles; drains (oclsph.hpp:693-706) erase
+    fluid inside the sphere of radius width and keep obstacles."""
+    from libfluid_amd._abi import particle_dtype
+    import types
+    h, scale = 0.1, 0.5
+    spacing = np.float32(h) * np.float32(scale) / np.float32(2)
+    src = lf.Source(centre=(1.0, 2.0, 3.0), velocity=(0.5, 0.0, -1.0), rate=18, tag=3,
+                    colour=0x11223344)
+    xs = np.zeros(3, dtype=particle_dtype())
+    xs["position"] = [(0.0, 0.0, 0.0), (5.0, 5.0, 5.0), (0.05, 0.0, 0.0)]
+    xs["type"] = [lf.LF_FLUID, lf.LF_FLUID, lf.LF_OBSTACLE]
+    cfg = types.SimpleNamespace(sources=[src], drains=[], scale=scale)
+    out = lf._apply_sources_drains(cfg, xs, h)
+    assert len(out) == 3 + 4 * 5
+    new = out[3:]
+    assert (new["type"] == lf.LF_FLUID).all() and (new["mass"] == 1).all()
+    assert (new["colour"] == 0x11223344).all() and (new["id"] == 3).all()
+    # x outer, z inner; width 4 in x, depth 5 in z; y on the centre
+    xsr = np.round((new["position"][:, 0] - 1.0) / spacing + 2.0).astype(int)
+    zsr = np.round((new["position"][:, 2] - 3.0) / spacing + 2.5).astype(int)
+    assert xsr.tolist() == [x for x in range(4) for _ in range(5)]
+    assert zsr.tolist() == [z for _ in range(4) for z in range(5)]
+    assert (new["position"][:, 1] == np.float32(2.0)).all()
+    assert np.allclose(new["position"][:, 0].mean(), 1.0 - spacing / 2, atol=1e-6)
+    # the lattice's centre of mass sits half a spacing below the centre in x
+    # (width 4 is even) and in z (offset depth/2 = 2.5 spacings, 5 points)
+    assert np.allclose(new["position"][:, 2].mean(), 3.0 - spacing / 2, atol=1e-6)
+    drain = lf.Drain(centre=(0.0, 0.0, 0.0), width=0.1, depth=0.0)
+    cfg = types.SimpleNamespace(sources=[], drains=[drain], scale=scale)
+    out = lf._apply_sources_drains(cfg, xs, h)
+    # the fluid particle at the centre goes, the far one and the obstacle stay
+    assert out["position"].tolist() == [[5.0, 5.0, 5.0], [np.float32(0.05), 0.0, 0.0]]
+    # a distance exactly equal to width is kept (strict <)
+    drain = lf.Drain(centre=(5.0, 5.0, 4.5), width=0.5, depth=0.0)
+    cfg = types.SimpleNamespace(sources=[], drains=[drain], scale=scale)
+    assert len(lf._apply_sources_drains(cfg, xs, h)) == 3
