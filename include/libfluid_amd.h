@@ -202,6 +202,27 @@ int lf_set_drains(lf_ctx* ctx, size_t n, const lf_drain* drains);
 int lf_set_queries(lf_ctx* ctx, size_t n, const float* points);
 int lf_query_results(lf_ctx* ctx, size_t n, uint32_t* counts, uint32_t* colours);
 
+/* ---- the memory-mapped file wire format ----------------------------------- */
+/* Blocks of the packed little-endian records libfluid's sample application
+ * writes to particles.mmf and triangles.mmf for its renderer
+ * (samples/structures.hpp:391-425; records per structures.hpp:73-85 and
+ * 234-258; include/fluid/wire.hpp restates the whole format): a 24-byte
+ * header {int64 timestamp, u64 entries, u64 written}, then the records. The
+ * records are packed on the device and copied into `dst` (host memory, e.g. a
+ * shared mapping of the file); the header is stored with written = 0 before
+ * the copy and with written = entries after it. *bytes receives the block
+ * size; dst = NULL only asks for it; a smaller capacity is LF_ERR_CAPACITY
+ * and writes nothing.
+ * lf_wire_particles: the state in its current (sorted) order; id = the
+ * particle's upload index (the order array of lf_state_download).
+ * lf_wire_triangles: the last lf_mc_emit output, normals 0 and colours
+ * 0xFFFFFFFF when it had no attributes (as lf_mc_download). */
+#define LF_WIRE_HEADER_BYTES 24
+#define LF_WIRE_PARTICLE_BYTES 44
+#define LF_WIRE_MESH_TRIANGLE_BYTES 84
+int lf_wire_particles(lf_ctx* ctx, void* dst, size_t capacity, int64_t timestamp, size_t* bytes);
+int lf_wire_triangles(lf_ctx* ctx, void* dst, size_t capacity, int64_t timestamp, size_t* bytes);
+
 /* ---- the PBF step, stage by stage --------------------------------------- */
 /* Predict (advectAndCopy, oclsph.hpp:289-325), Morton key
  * (computeBoundAndZindex, oclsph.hpp:327-387), stable radix sort (replaces the

@@ -201,6 +201,28 @@ class Context:
         self._check(self._lib.lf_query_results(self._h, n, _ptr(counts), _ptr(colours)))
         return counts, colours
 
+    def _wire(self, fn, buf, timestamp):
+        need = ctypes.c_size_t()
+        ts = ctypes.c_int64(int(timestamp))
+        if buf is None:
+            self._check(fn(self._h, None, 0, ts, ctypes.byref(need)))
+            out = np.zeros(need.value, np.uint8)
+            self._check(fn(self._h, _ptr(out), out.nbytes, ts, ctypes.byref(need)))
+            return out.tobytes()
+        arr = np.frombuffer(buf, np.uint8)  # a writable buffer: bytearray, mmap, numpy
+        self._check(fn(self._h, arr.ctypes.data, arr.nbytes, ts, ctypes.byref(need)))
+        return need.value
+
+    def wire_particles(self, buf=None, timestamp=0):
+        """The particle block of the mmap wire format, packed on the device.
+        With buf (a writable buffer such as an mmap) it is written there and
+        the block size returned; without, the block is returned as bytes."""
+        return self._wire(self._lib.lf_wire_particles, buf, timestamp)
+
+    def wire_triangles(self, buf=None, timestamp=0):
+        """The meshTriangle block of the last marching-cubes emit."""
+        return self._wire(self._lib.lf_wire_triangles, buf, timestamp)
+
     # -- stages ---------------------------------------------------------------
     def grid_build(self, p):
         self._check(self._lib.lf_grid_build(self._h, ctypes.byref(p)))

@@ -321,7 +321,6 @@ static int pbfStepOnce(Ctx* c) {
 using lf::Ctx;
 using lf::fail;
 
-struct lf_ctx : lf::Ctx {};
 
 extern "C" {
 
@@ -398,6 +397,7 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->queryPts);
   lf::release(c->queryOut);
   lf::release(c->compactBlock);
+  lf::release(c->wireStage);
   lf::release(c->field);
   lf::release(c->mcNormal);
   lf::release(c->mcColour);

@@ -371,6 +371,7 @@ struct Ctx {
   bool queriesAnswered = false;
   uint32_t nextId = 0;          // upload index of the next appended particle
   DevBuf<uint32_t> compactBlock;  // per-block counts -> offsets of the drain compaction
+  DevBuf<uint32_t> wireStage;     // packed wire records before the copy to the host
   std::vector<lf_well> hostWells;
 
   bool gridValid = false;
@@ -565,4 +566,7 @@ constexpr int kStatSlots = kPassSlots + 1;
 int densityCheck(Ctx* c, int pass);
 int diagState(Ctx* c);
 
-}  // namespace lf
+}  // namespace lf// The opaque handle of the C ABI is the context itself.
+struct lf_ctx : lf::Ctx {};
+
+
