@@ -44,7 +44,7 @@ constexpr int kBlockLambda = LF_BLOCK_LAMBDA;  // list entries in flight per lan
 constexpr int kBlockDelta = LF_BLOCK_DELTA;
 constexpr int kBlockXsph = LF_BLOCK_XSPH;
 #ifndef LF_BLOCK_BUILD
-#define LF_BLOCK_BUILD 1
+#define LF_BLOCK_BUILD 4
 #endif
 constexpr int kBuildBlock = LF_BLOCK_BUILD;  // candidates loaded together (list build)
 
@@ -101,6 +101,9 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ table,
                                                    const uint8_t* __restrict__ T, Lists L) {
+  // per lane, the entries of the group being filled: win[j * BS + lane] holds
+  // entry (group start + j); lanes side by side, so no bank conflicts
+  __shared__ uint32_t win[kNbrGroup * BS];
   if (blockIdx.x == 0 && threadIdx.x < 4) L.maxDisp[threadIdx.x] = 0u;
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
@@ -111,10 +114,7 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
     }
     const float4 pa = L.P0[a];
     uint32_t cnt = 0;
-    uint32_t w[kNbrGroup];  // w[j]: entry (cnt - kNbrGroup + j) once shifted in
-#pragma unroll
-    for (int j = 0; j < int(kNbrGroup); ++j) w[j] = 0u;
-    forEachNeighbourCell(keys[a], table + c.keyBase, c.G, [&](uint32_t s, uint32_t e) {
+    forEachNeighbourCellRolled(keys[a], table + c.keyBase, c.G, [&](uint32_t s, uint32_t e) {
       for (uint32_t b0 = s; b0 < e; b0 += kBuildBlock) {
         float4 pb[kBuildBlock];  // candidates loaded together (indices clamped into the cell)
 #pragma unroll
@@ -124,31 +124,23 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
           const uint32_t b = b0 + k;
           if (!(b < e) || !(dist2(pa.x - pb[k].x, pa.y - pb[k].y, pa.z - pb[k].z) <= c.listR2))
             continue;
-#pragma unroll
-          for (int j = 0; j + 1 < int(kNbrGroup); ++j) w[j] = w[j + 1];
-          w[kNbrGroup - 1] = b;
+          win[(cnt % kNbrGroup) * BS + threadIdx.x] = b;
           ++cnt;
           if (cnt % kNbrGroup == 0u && cnt <= kMaxNbr) {
             uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, cnt - kNbrGroup));
-            g[0] = make_uint4(w[0], w[1], w[2], w[3]);
-            g[1] = make_uint4(w[4], w[5], w[6], w[7]);
+            g[0] = make_uint4(win[0 * BS + threadIdx.x], win[1 * BS + threadIdx.x],
+                              win[2 * BS + threadIdx.x], win[3 * BS + threadIdx.x]);
+            g[1] = make_uint4(win[4 * BS + threadIdx.x], win[5 * BS + threadIdx.x],
+                              win[6 * BS + threadIdx.x], win[7 * BS + threadIdx.x]);
           }
         }
       }
     });
-    // the last r = cnt % 8 entries are w[8 - r] .. w[7]: shift them down to
-    // w[0] .. w[r - 1] (constant register indices) and store them
+    // the last cnt % 8 entries
     const uint32_t r = cnt % kNbrGroup;
     if (r != 0u && cnt <= kMaxNbr) {
-#pragma unroll
-      for (uint32_t sh = 0; sh < kNbrGroup; ++sh)
-        if (sh < kNbrGroup - r)
-#pragma unroll
-          for (int j = 0; j + 1 < int(kNbrGroup); ++j) w[j] = w[j + 1];
       uint32_t* g = L.nbr + nbrSlot(a, cnt - r);
-#pragma unroll
-      for (int j = 0; j < int(kNbrGroup); ++j)
-        if (uint32_t(j) < r) g[j] = w[j];
+      for (uint32_t j = 0; j < r; ++j) g[j] = win[j * BS + threadIdx.x];
     }
     L.count[a] = cnt;
   });

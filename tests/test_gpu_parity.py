@@ -144,6 +144,14 @@ def test_step_1m_bit_exact(ctx):
     g = ctx.download()
     _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position 1M")
     _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity 1M")
+    # the 256^3 density field of the stepped state (the staged-tile path with
+    # its out-of-reach cells cut, and the global-memory path for crowded tiles)
+    m = sc.mc()
+    ctx.mc_sample(p, m)
+    field = ctx.read(LF_ARR_FIELD)
+    ofield = o.mc_density(m.samples)
+    assert (field > 0).sum() > 1_000_000
+    _eq(field.astype(np.float64).reshape(ofield.shape), ofield, "density field 1M 256^3")
 
 
 def test_density_mc_bit_exact(ctx):
