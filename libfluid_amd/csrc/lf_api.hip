@@ -393,6 +393,10 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->sortHist);
   lf::release(c->wells);
   lf::release(c->colliders);
+  lf::release(c->colliderBounds);
+  lf::release(c->colliderCellStart);
+  lf::release(c->colliderCellTris);
+  lf::release(c->colliderCounts);
   lf::release(c->drains);
   lf::release(c->queryPts);
   lf::release(c->queryOut);
@@ -597,42 +601,7 @@ int lf_set_colliders(lf_ctx* c, size_t n, const lf_triangle* tris) {
   if (n > 0 && !tris) return fail(c, LF_ERR_ARG, "lf_set_colliders: null triangles");
   if (n >= (size_t(1) << 31)) return fail(c, LF_ERR_ARG, "lf_set_colliders: too many triangles");
   LF_HIP(c, hipSetDevice(c->device));
-  std::vector<lf::ColliderTri> host(n);
-  for (size_t i = 0; i < n; ++i) {
-    const lf_triangle& t = tris[i];
-    lf::ColliderTri& T = host[i];
-    float e1[3], e2[3], nv[3];
-    for (int k = 0; k < 3; ++k) {
-      T.a[k] = t.v0[k];
-      e1[k] = t.v1[k] - t.v0[k];  // b - a
-      e2[k] = t.v2[k] - t.v0[k];  // c - a
-    }
-    nv[0] = e1[1] * e2[2] - e1[2] * e2[1];
-    nv[1] = e1[2] * e2[0] - e1[0] * e2[2];
-    nv[2] = e1[0] * e2[1] - e1[1] * e2[0];
-    const float len = std::sqrt(lf::dot3(nv, nv));
-    for (int k = 0; k < 3; ++k) {
-      T.nn[k] = nv[k] / len;
-      T.v0[k] = e2[k];
-      T.v1[k] = e1[k];
-    }
-    T.dn = lf::dot3(T.nn, T.a);
-    T.dot00 = lf::dot3(T.v0, T.v0);
-    T.dot01 = lf::dot3(T.v0, T.v1);
-    T.dot11 = lf::dot3(T.v1, T.v1);
-    T.invDenom = 1.0f / (T.dot00 * T.dot11 - T.dot01 * T.dot01);
-    T.pad[0] = T.pad[1] = T.pad[2] = 0.0f;
-  }
-  if (n > 0) {
-    int rc = lf::ensure(c, c->colliders, n, "colliders");
-    if (rc) return rc;
-    LF_HIP(c, hipMemcpyAsync(c->colliders.p, host.data(), n * sizeof(lf::ColliderTri),
-                             hipMemcpyHostToDevice, c->stream));
-    LF_HIP(c, hipStreamSynchronize(c->stream));
-  }
-  lf::dropGraph(c);  // the captured delta launches hold the old count
-  c->nColliders = uint32_t(n);
-  return LF_OK;
+  return lf::setColliders(c, n, tris);
 }
 
 int lf_grid_build(lf_ctx* c, const lf_pbf_params* p) {

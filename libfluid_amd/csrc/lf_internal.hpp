@@ -317,34 +317,73 @@ __host__ __device__ inline float dot3(const float* u, const float* v) {
   return __builtin_fmaf(u[2], v[2], __builtin_fmaf(u[1], v[1], u[0] * v[0]));
 }
 
-// collideTriangle2 (oclsph_collision.h:25-58): the first triangle, in order,
-// whose plane lies within 40 world units of p (t = dn - nn.p) and whose
-// projection p0 = p + nn t falls inside it (inTrig, :7-22: u >= 0, v >= 0,
-// u + v < 1) moves p to p0 and reflects v about the plane (v - nn (2 v.nn)).
-__device__ inline bool collideTriangles(const ColliderTri* __restrict__ tris, uint32_t n,
-                                        float p[3], float v[3]) {
-  for (uint32_t i = 0; i < n; ++i) {
-    const ColliderTri& T = tris[i];
-    const float t = T.dn - dot3(T.nn, p);
-    if (!(-40.0f <= t && t <= 40.0f)) continue;
-    float p0[3], v2[3];
+// collideTriangle2 (oclsph_collision.h:25-58) for one triangle: when its
+// plane lies within 40 world units of p (t = dn - nn.p) and the projection
+// p0 = p + nn t falls inside it (inTrig, :7-22: u >= 0, v >= 0, u + v < 1), p
+// moves to p0 and v reflects about the plane (v - nn (2 v.nn)).
+__device__ inline bool tryTriangle(const ColliderTri& T, float p[3], float v[3]) {
+  const float t = T.dn - dot3(T.nn, p);
+  if (!(-40.0f <= t && t <= 40.0f)) return false;
+  float p0[3], v2[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      p0[k] = __builtin_fmaf(T.nn[k], t, p[k]);
-      v2[k] = p0[k] - T.a[k];
-    }
-    const float dot02 = dot3(T.v0, v2), dot12 = dot3(T.v1, v2);
-    const float u = (T.dot11 * dot02 - T.dot01 * dot12) * T.invDenom;
-    const float w = (T.dot00 * dot12 - T.dot01 * dot02) * T.invDenom;
-    if (!(u >= 0.0f && w >= 0.0f && u + w < 1.0f)) continue;
-    const float s = 2.0f * dot3(v, T.nn);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      p[k] = p0[k];
-      v[k] = __builtin_fmaf(-T.nn[k], s, v[k]);
-    }
-    return true;
+  for (int k = 0; k < 3; ++k) {
+    p0[k] = __builtin_fmaf(T.nn[k], t, p[k]);
+    v2[k] = p0[k] - T.a[k];
   }
+  const float dot02 = dot3(T.v0, v2), dot12 = dot3(T.v1, v2);
+  const float u = (T.dot11 * dot02 - T.dot01 * dot12) * T.invDenom;
+  const float w = (T.dot00 * dot12 - T.dot01 * dot02) * T.invDenom;
+  if (!(u >= 0.0f && w >= 0.0f && u + w < 1.0f)) return false;
+  const float s = 2.0f * dot3(v, T.nn);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    p[k] = p0[k];
+    v[k] = __builtin_fmaf(-T.nn[k], s, v[k]);
+  }
+  return true;
+}
+
+// The colliders of a delta pass. The first triangle in list order that hits
+// wins (oclsph_collision.h:45-57). With a grid, the point's cell lists, in
+// ascending order, every triangle that can hit anywhere in the cell (a
+// conservative superset), so the first hit found there is the first hit of
+// the whole list; points outside the grid scan the whole list.
+struct ColliderGrid {
+  const ColliderTri* tris;
+  const uint32_t* cellStart;  // cells + 1 offsets into cellTris
+  const uint32_t* cellTris;   // triangle indices, ascending within a cell
+  float origin[3];            // world units
+  float invCell;
+  uint32_t dims[3];
+  uint32_t n;                 // triangles (0: no colliders)
+  uint32_t useGrid;           // 0: scan the whole list
+};
+
+__device__ inline bool collide(const ColliderGrid& g, float p[3], float v[3]) {
+  if (g.useGrid) {
+    const float qx = (p[0] - g.origin[0]) * g.invCell;
+    const float qy = (p[1] - g.origin[1]) * g.invCell;
+    const float qz = (p[2] - g.origin[2]) * g.invCell;
+    if (qx >= 0.0f && qx < float(g.dims[0]) && qy >= 0.0f && qy < float(g.dims[1]) &&
+        qz >= 0.0f && qz < float(g.dims[2])) {
+      const uint32_t cell =
+          (uint32_t(qz) * g.dims[1] + uint32_t(qy)) * g.dims[0] + uint32_t(qx);
+      const uint32_t e = g.cellStart[cell + 1u];
+      uint32_t j = g.cellStart[cell];
+      if (j >= e) return false;
+      // the next triangle's loads are issued before the current one is tested
+      ColliderTri next = g.tris[g.cellTris[j]];
+      while (true) {
+        const ColliderTri cur = next;
+        const bool more = ++j < e;
+        if (more) next = g.tris[g.cellTris[j]];
+        if (tryTriangle(cur, p, v)) return true;
+        if (!more) return false;
+      }
+    }
+  }
+  for (uint32_t i = 0; i < g.n; ++i)
+    if (tryTriangle(g.tris[i], p, v)) return true;
   return false;
 }
 
@@ -389,6 +428,12 @@ struct Ctx {
   int nWells = 0;
   DevBuf<ColliderTri> colliders;
   uint32_t nColliders = 0;
+  std::vector<lf_triangle> hostColliders;   // as given, for the grid's inclusion tests
+  DevBuf<double> colliderBounds;            // per triangle: inclusion-test record
+  DevBuf<uint32_t> colliderCellStart, colliderCellTris, colliderCounts;
+  ColliderGrid colliderGrid{};              // device pointers filled in when built
+  bool colliderGridValid = false;
+  float colliderGridBox[6] = {0, 0, 0, 0, 0, 0};  // scene bounds the grid was built for
   std::vector<lf_source> sources;
   std::vector<lf_drain> hostDrains;
   DevBuf<lf_drain> drains;
@@ -584,7 +629,9 @@ int mcClassifyCount(Ctx* c, const lf_mc_params& m);
 int mcEmit(Ctx* c, const lf_mc_params& m);
 int copyProbe(Ctx* c, size_t bytes, int reps, float* ms);  // lf_grid.hip
 int applyDrains(Ctx* c);                      // lf_grid.hip: stable erase, updates n
-int answerQueries(Ctx* c);                    // lf_grid.hip: after the grid build
+int answerQueries(Ctx* c);
+int setColliders(Ctx* c, size_t n, const lf_triangle* tris);
+int prepareColliderGrid(Ctx* c);                    // lf_grid.hip: after the grid build
 // lf_diag.hip: FP64 per-scene reductions. densityCheck runs after lambda pass
 // `pass` (convergence); diagState reduces the current state into the last slot.
 constexpr int kSumWidth = 8;   // count, mass, momentum x3, sum |e|, max |e|, unused

@@ -302,8 +302,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
                                               const float4* __restrict__ PL,
                                               const float4* __restrict__ X, Lists L, int fresh,
                                               float4* __restrict__ Pout, float4* __restrict__ V,
-                                              const ColliderTri* __restrict__ mesh,
-                                              uint32_t meshN) {
+                                              ColliderGrid mesh) {
   __shared__ float waveMax[BS / 64];
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
@@ -330,10 +329,10 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
         float pos[3], o[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) pos[k] = (p[k] + acc[k]) * c.scale;
-        if (meshN) {
+        if (mesh.n) {
           const float4 va = V[a];
           float vel[3] = {va.x, va.y, va.z};
-          if (collideTriangles(mesh, meshN, pos, vel))
+          if (collide(mesh, pos, vel))
             V[a] = make_float4(vel[0], vel[1], vel[2], va.w);  // velocity[a] = resp.velocity
         }
 #pragma unroll
@@ -558,6 +557,8 @@ int pbfLambda(Ctx* c, bool skipDone) {
 
 int pbfDelta(Ctx* c, bool skipDone) {
   if (c->n == 0) return LF_OK;
+  int grc = prepareColliderGrid(c);
+  if (grc) return grc;
   const Work W = workOf(c, kSlotDelta, skipDone);
   const bool fastMath = (c->consts.flags & LF_FLAG_FAST_MATH) != 0;
   const int rc = withBS(c, [&](auto bs) {
@@ -567,8 +568,7 @@ int pbfDelta(Ctx* c, bool skipDone) {
                     dim3(decltype(bs)::value), 0, W, (const uint32_t*)c->localKey.p,
                     (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
                     (const float4*)c->PL.p, (const float4*)c->X[c->cur].p, listsOf(c),
-                    int(c->listFresh), c->P.p, c->V[c->cur].p,
-                    (const ColliderTri*)c->colliders.p, c->nColliders);
+                    int(c->listFresh), c->P.p, c->V[c->cur].p, c->colliderGrid);
     });
   });
   c->listFresh = false;  // p* now differs from the predicted positions

@@ -434,3 +434,31 @@ def test_mesh_collider_bit_exact(ctx, steps):
     o2 = _pair(ctx, sc, p)
     ctx.step(p, steps)
     assert not np.array_equal(ctx.download()["position"], g["position"])
+
+
+@pytest.mark.parametrize("steps", [1, 4])
+def test_mesh_collider_grid_bit_exact(ctx, steps):
+    """A 1,280-triangle sphere cutting into the block plus the ramp: the delta
+    pass looks triangles up through the collider grid (per-cell lists of the
+    triangles whose +-40 prism may reach the cell); the oracle scans the whole
+    list. First hits, positions and velocities must agree exactly."""
+    sc = scenes.dam_break_8k()
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
+    o = _pair(ctx, sc, p)
+    mesh = np.concatenate([_ramp_mesh(), scenes.icosphere((0.9, 0.35, 0.5), 0.25, 3)])
+    assert len(mesh) == 3 + 1280
+    ctx.set_colliders(mesh)
+    o.set_colliders(mesh)
+    try:
+        ctx.profile(True)
+        ctx.profile_reset()
+        ctx.step(p, steps)
+        stats = ctx.kernel_stats()
+        ctx.profile(False)
+        assert "collider_grid_fill" in stats  # the grid was built and is in use
+        o.step(steps)
+        g = ctx.download()
+        _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position")
+        _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity")
+    finally:
+        ctx.set_colliders(np.zeros((0, 3, 3), np.float32))
