@@ -56,7 +56,19 @@ struct Lattice {
 // a local cell table, so the per-sample candidate loop reads LDS; a tile whose
 // halo holds no particle writes zeros and exits. Tiles whose halo does not fit
 // (coarse lattices, crowded cells) read global memory instead, same order.
-constexpr int kTX = 4, kTY = 8, kTZ = 8;  // 256 samples
+// 8 x 8 x 4 samples (x, y, z) per tile: measured fastest of 12 shapes at
+// 256^3 on the 1M dam break (373-391 us against 597-764 us for 4 x 8 x 8)
+#ifndef LF_MC_TILE_X
+#define LF_MC_TILE_X 8
+#define LF_MC_TILE_Y 8
+#define LF_MC_TILE_Z 4
+#endif
+#ifndef LF_MC_UNROLL
+#define LF_MC_UNROLL 4
+#endif
+constexpr int kTX = LF_MC_TILE_X, kTY = LF_MC_TILE_Y, kTZ = LF_MC_TILE_Z;  // 256 samples
+static_assert(kTX * kTY * kTZ == kThreads, "one sample per thread");
+constexpr uint32_t kUnroll = LF_MC_UNROLL;  // staged particles in flight per sample
 constexpr int kHaloCells = 512;
 constexpr int kStageCap = 2048;           // staged particles (32 KB)
 
@@ -109,6 +121,31 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
     fits = hx * hy * hz <= uint32_t(kHaloCells);
   }
   const uint32_t nh = fits ? hx * hy * hz : 0;
+  if (fits) {
+    // Quick emptiness test: an aligned 4x4x4 block of cells is one run of 64
+    // Morton codes, so table[end] - table[start] counts its particles (codes
+    // up to G - 1, the cells that hold any). A tile whose halo lies in blocks
+    // holding none writes zeros without staging anything.
+    const uint32_t b0[3] = {(lo[0] - 1) >> 2, (lo[1] - 1) >> 2, (lo[2] - 1) >> 2};
+    const uint32_t nb[3] = {((hi[0] + 1) >> 2) - b0[0] + 1, ((hi[1] + 1) >> 2) - b0[1] + 1,
+                            ((hi[2] + 1) >> 2) - b0[2] + 1};
+    bool any = false;
+    if (threadIdx.x < nb[0] * nb[1] * nb[2]) {
+      const uint32_t i = threadIdx.x;
+      const uint32_t base = mortonEncode(4u * (b0[0] + i % nb[0]), 4u * (b0[1] + (i / nb[0]) % nb[1]),
+                                         4u * (b0[2] + i / (nb[0] * nb[1])));
+      const uint32_t last = c.G - 1u;
+      any = table[min(base + 64u, last)] != table[min(base, last)];
+    }
+    if (!__syncthreads_or(any)) {
+      const uint32_t lx = threadIdx.x / (kTY * kTZ), ly = (threadIdx.x / kTZ) % kTY,
+                     lz = threadIdx.x % kTZ;
+      const uint32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
+      if (x < L.S[0] && y < L.S[1] && z < L.S[2])
+        field[(uint64_t(x) * L.S[1] + y) * L.S[2] + z] = 0.0f;
+      return;
+    }
+  }
   // count the halo cells' particles (per thread: cells tid, tid+256)
   uint32_t myCount[2] = {0, 0};
   for (int r = 0; r < 2; ++r) {
@@ -169,13 +206,21 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
       if (h < nh && myCount[r] > 0) {
         const uint32_t ix = h % hx, iy = (h / hx) % hy, iz = h / (hx * hy);
         const uint32_t code = mortonEncode(lo[0] - 1 + ix, lo[1] - 1 + iy, lo[2] - 1 + iz);
-        const uint32_t s0 = table[code], dst = cellStart[h];
-        for (uint32_t j = 0; j < myCount[r]; ++j) {
-          const float4 q = P[s0 + j];
-          stX[dst + j] = q.x;
-          stY[dst + j] = q.y;
-          stZ[dst + j] = q.z;
-          stM[dst + j] = q.w;
+        const uint32_t s0 = table[code], dst = cellStart[h], cnt = myCount[r];
+        // four loads in flight per round (the last round's extra lanes load
+        // the cell's last particle again and store nothing)
+        for (uint32_t j = 0; j < cnt; j += 4) {
+          float4 q[4];
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u) q[u] = P[s0 + min(j + u, cnt - 1u)];
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u)
+            if (j + u < cnt) {
+              stX[dst + j + u] = q[u].x;
+              stY[dst + j + u] = q[u].y;
+              stZ[dst + j + u] = q[u].z;
+              stM[dst + j + u] = q[u].w;
+            }
         }
       }
     }
@@ -205,15 +250,15 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
         const uint32_t row = ((bz + dz) * hy + (by + dy)) * hx + bx;
         const uint32_t e = cellStart[row + 3];
         uint32_t b = cellStart[row];
-        // four staged particles in flight, terms added in order
-        for (; b + 4 <= e; b += 4) {
-          float t[4];
+        // kUnroll staged particles in flight, terms added in order
+        for (; b + kUnroll <= e; b += kUnroll) {
+          float t[kUnroll];
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (uint32_t u = 0; u < kUnroll; ++u)
             t[u] = densityTerm(c, make_float4(stX[b + u], stY[b + u], stZ[b + u], stM[b + u]),
                                px, py, pz);
 #pragma unroll
-          for (int u = 0; u < 4; ++u) rho += t[u];
+          for (uint32_t u = 0; u < kUnroll; ++u) rho += t[u];
         }
         for (; b < e; ++b)
           rho += densityTerm(c, make_float4(stX[b], stY[b], stZ[b], stM[b]), px, py, pz);
