@@ -198,6 +198,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ cou
   const uint32_t per = (nb + 1023) / 1024;
   const uint32_t lo = threadIdx.x * per, hi = min(lo + per, nb);
   uint32_t s = 0;
+#pragma unroll 8
   for (uint32_t i = lo; i < hi; ++i) s += counts[i];
   uint32_t x = s;
   const int lane = __lane_id(), w = threadIdx.x >> 6;

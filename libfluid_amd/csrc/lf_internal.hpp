@@ -231,30 +231,42 @@ __device__ inline void forEachNeighbourCell(uint32_t key, const uint32_t* __rest
   }
 }
 
-// The same 27 cells in the same order with the cell loop kept rolled: the
+// The same 27 cells in the same order with the cell loop kept rolled (the
 // body is emitted once instead of 27 times, for callers whose body is large
-// and runs for every cell (the list build).
+// and runs for every cell: the list build), and the next cell's table
+// entries loaded before the current cell is visited, so their latency
+// overlaps the visit.
 template <typename Fn>
 __device__ inline void forEachNeighbourCellRolled(uint32_t key, const uint32_t* __restrict__ table,
                                                   uint32_t G, Fn&& fn) {
-  const uint32_t xd = dilatedDec(key, kMortonX), xc = key & kMortonX,
-                 xi = dilatedInc(key, kMortonX);
-  const uint32_t yd = dilatedDec(key, kMortonY), yc = key & kMortonY,
-                 yi = dilatedInc(key, kMortonY);
-  const uint32_t zd = dilatedDec(key, kMortonZ), zc = key & kMortonZ,
-                 zi = dilatedInc(key, kMortonZ);
+  const uint32_t xs[3] = {dilatedDec(key, kMortonX), key & kMortonX, dilatedInc(key, kMortonX)};
+  const uint32_t ys[3] = {dilatedDec(key, kMortonY), key & kMortonY, dilatedInc(key, kMortonY)};
+  const uint32_t zs[3] = {dilatedDec(key, kMortonZ), key & kMortonZ, dilatedInc(key, kMortonZ)};
+  auto codeOf = [&](uint32_t i) {
+    const uint32_t ix = i % 3u, iy = (i / 3u) % 3u, iz = i / 9u;
+    return (ix == 0 ? xs[0] : (ix == 1 ? xs[1] : xs[2])) |
+           (iy == 0 ? ys[0] : (iy == 1 ? ys[1] : ys[2])) |
+           (iz == 0 ? zs[0] : (iz == 1 ? zs[1] : zs[2]));
+  };
+  uint32_t code = codeOf(0);
+  bool valid = code + 1u < G;
+  uint32_t s = valid ? table[code] : 0u, e = valid ? table[code + 1u] : 0u;
 #pragma unroll 1
-  for (int iz = 0; iz < 3; ++iz) {
-    const uint32_t z = iz == 0 ? zd : (iz == 1 ? zc : zi);
-#pragma unroll 1
-    for (int iy = 0; iy < 3; ++iy) {
-      const uint32_t yz = (iy == 0 ? yd : (iy == 1 ? yc : yi)) | z;
-#pragma unroll 1
-      for (int ix = 0; ix < 3; ++ix) {
-        const uint32_t code = (ix == 0 ? xd : (ix == 1 ? xc : xi)) | yz;
-        if (code + 1u < G) fn(table[code], table[code + 1u]);
+  for (uint32_t i = 0; i < 27u; ++i) {
+    uint32_t sN = 0u, eN = 0u;
+    bool validN = false;
+    if (i + 1u < 27u) {
+      const uint32_t codeN = codeOf(i + 1u);
+      validN = codeN + 1u < G;
+      if (validN) {
+        sN = table[codeN];
+        eN = table[codeN + 1u];
       }
     }
+    if (valid) fn(s, e);
+    s = sN;
+    e = eN;
+    valid = validN;
   }
 }
 
@@ -502,6 +514,7 @@ struct Ctx {
   bool mcSampled = false, mcCounted = false, mcEmitted = false;
   int mcAttr = 0;
   bool keepCases = false;
+  float mcIso = 0.0f;  // isolevel of the last classify pass (its cases feed the emit)
 
   // profiling
   bool prof = false;

@@ -417,7 +417,7 @@ __global__ __launch_bounds__(kThreads) void k_mc_count(CubeGrid g, const float* 
     if (j > 0) q.advance(g, kThreads);
     if (m < g.cubes) {
       const uint32_t ci = cubeCase(g, field, q, iso);
-      if (cases) cases[m] = uint8_t(ci);
+      cases[m] = uint8_t(ci);
       sum += cTab.ntri[ci];
     }
   }
@@ -441,6 +441,7 @@ __global__ __launch_bounds__(1024) void k_mc_scan(uint32_t* __restrict__ counts,
   const uint32_t per = (nb + 1023) / 1024;
   const uint32_t lo = threadIdx.x * per, hi = min(lo + per, nb);
   uint32_t s = 0;
+#pragma unroll 8
   for (uint32_t i = lo; i < hi; ++i) s += counts[i];
   uint32_t x = s;
   const int lane = __lane_id(), w = threadIdx.x >> 6;
@@ -457,6 +458,7 @@ __global__ __launch_bounds__(1024) void k_mc_scan(uint32_t* __restrict__ counts,
     tot += sm[i];
   }
   uint32_t run = wb + x - s;
+#pragma unroll 8
   for (uint32_t i = lo; i < hi; ++i) {
     const uint32_t t = counts[i];
     counts[i] = run;
@@ -530,9 +532,15 @@ __device__ inline EdgeVertex edgeVertex(const EmitArgs& A, const float* __restri
 
 // mc_eval (oclsph_kernel.h:599-735): edge intersections by linear
 // interpolation t = (iso - v0)/(v1 - v0), mix(p0, p1, t); triangles of
-// TriTable[ci] written at the scanned offset. Blocks without triangles exit
-// at once; each emitted vertex is interpolated where it is written.
+// TriTable[ci] written at the scanned offset, in cube order. Blocks without
+// triangles exit at once. The case indices come from the classify pass (one
+// byte per cube). Each cube with triangles stamps its block-local index over
+// its slots in LDS, then thread k emits the block's triangle k: the threads
+// share the work evenly and consecutive threads write consecutive records.
+constexpr uint32_t kMaxBlockTriangles = kCubesPerBlock * 5;  // at most 5 per cube
+
 __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* __restrict__ field,
+                                                      const uint8_t* __restrict__ cases,
                                                       const float4* __restrict__ normals,
                                                       const uint32_t* __restrict__ colours,
                                                       const uint32_t* __restrict__ blockOffset,
@@ -542,60 +550,72 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* _
                                                       float* __restrict__ triNrm,
                                                       uint32_t* __restrict__ triCol) {
   __shared__ uint32_t sm[kThreads / 64];
+  __shared__ uint16_t owner[kMaxBlockTriangles];  // block-local cube of each triangle
+  __shared__ uint8_t triInCube[kMaxBlockTriangles];  // its index within the cube
+  __shared__ uint8_t caseOf[kCubesPerBlock];
   const uint64_t first = blockOffset[blockIdx.x];
   const uint64_t next = (blockIdx.x + 1 < nBlocks) ? blockOffset[blockIdx.x + 1] : total[0];
   if (next == first) return;  // uniform: no triangle in this block
-  uint64_t run = first;
-  const uint32_t m0 = blockIdx.x * kCubesPerBlock + threadIdx.x;
-  CubeCursor q(A.g, min(m0, A.g.cubes - 1u));
-  for (int j = 0; j < kCubesPerThread; ++j) {
-    const uint32_t m = m0 + uint32_t(j) * kThreads;
-    if (j > 0) q.advance(A.g, kThreads);
-    uint32_t ci = 0;
-    if (m < A.g.cubes) ci = cubeCase(A.g, field, q, A.iso);
+  const uint32_t base = blockIdx.x * kCubesPerBlock;
+  uint32_t run = 0;  // triangles of the block's earlier rounds
+  for (uint32_t j = 0; j < uint32_t(kCubesPerThread); ++j) {
+    const uint32_t ib = j * kThreads + threadIdx.x;  // block-local cube, cube order
+    const uint32_t m = base + ib;
+    const uint32_t ci = m < A.g.cubes ? uint32_t(cases[m]) : 0u;
+    caseOf[ib] = uint8_t(ci);
     const uint32_t nt = cTab.ntri[ci];
     uint32_t tot;
-    const uint32_t pre = blockPrefixSmall(nt, sm, &tot);
+    const uint32_t pre = run + blockPrefixSmall(nt, sm, &tot);
     for (uint32_t i = 0; i < nt; ++i) {
-      const uint64_t slot = run + pre + i;
-      if (slot >= A.cap) break;
-      for (int v = 0; v < 3; ++v) {
-        const int e = cTab.tri[ci][3 * i + v];
-        const EdgeVertex r = edgeVertex(A, field, q, e);
-        float* dst = triPos + slot * 9 + 3 * v;
-        dst[0] = r.p[0];
-        dst[1] = r.p[1];
-        dst[2] = r.p[2];
-        if (A.attributes) {
-          float na[3] = {0.0f, 0.0f, 0.0f}, nb[3] = {0.0f, 0.0f, 0.0f};
-          float ca[4], cb[4];
-          if (normals) {
-            const float4 n0 = normals[r.ia], n1 = normals[r.ib];
-            na[0] = n0.x; na[1] = n0.y; na[2] = n0.z;
-            nb[0] = n1.x; nb[1] = n1.y; nb[2] = n1.z;
-          }
-          if (colours) {
-            const uint32_t c0 = colours[r.ia], c1 = colours[r.ib];
-            for (int k = 0; k < 4; ++k) {
-              ca[k] = float((c0 >> (24 - 8 * k)) & 0xFFu);
-              cb[k] = float((c1 >> (24 - 8 * k)) & 0xFFu);
-            }
-          } else {
-            for (int k = 0; k < 4; ++k) ca[k] = cb[k] = 255.0f;
-          }
-          float* nd = triNrm + slot * 9 + 3 * v;
-          for (int k = 0; k < 3; ++k) nd[k] = __builtin_fmaf(nb[k] - na[k], r.t, na[k]);
-          uint32_t packed = 0;
-          for (int k = 0; k < 4; ++k) {
-            const float mv = __builtin_fmaf(cb[k] - ca[k], r.t, ca[k]);
-            const uint32_t u = !(mv > 0.0f) ? 0u : (mv >= 255.0f ? 255u : uint32_t(mv));
-            packed = (packed << 8) | u;
-          }
-          triCol[slot * 3 + v] = packed;
-        }
-      }
+      owner[pre + i] = uint16_t(ib);
+      triInCube[pre + i] = uint8_t(i);
     }
     run += tot;
+  }
+  __syncthreads();
+  const uint32_t T = run;  // == next - first
+  for (uint32_t k = threadIdx.x; k < T; k += kThreads) {
+    const uint64_t slot = first + k;
+    if (slot >= A.cap) break;
+    const uint32_t ib = owner[k], i = triInCube[k];
+    const uint32_t ci = caseOf[ib];
+    const CubeCursor q(A.g, base + ib);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const int e = cTab.tri[ci][3 * i + v];
+      const EdgeVertex r = edgeVertex(A, field, q, e);
+      float* dst = triPos + slot * 9 + 3 * v;
+      dst[0] = r.p[0];
+      dst[1] = r.p[1];
+      dst[2] = r.p[2];
+      if (A.attributes) {
+        float na[3] = {0.0f, 0.0f, 0.0f}, nb[3] = {0.0f, 0.0f, 0.0f};
+        float ca[4], cb[4];
+        if (normals) {
+          const float4 n0 = normals[r.ia], n1 = normals[r.ib];
+          na[0] = n0.x; na[1] = n0.y; na[2] = n0.z;
+          nb[0] = n1.x; nb[1] = n1.y; nb[2] = n1.z;
+        }
+        if (colours) {
+          const uint32_t c0 = colours[r.ia], c1 = colours[r.ib];
+          for (int k2 = 0; k2 < 4; ++k2) {
+            ca[k2] = float((c0 >> (24 - 8 * k2)) & 0xFFu);
+            cb[k2] = float((c1 >> (24 - 8 * k2)) & 0xFFu);
+          }
+        } else {
+          for (int k2 = 0; k2 < 4; ++k2) ca[k2] = cb[k2] = 255.0f;
+        }
+        float* nd = triNrm + slot * 9 + 3 * v;
+        for (int k2 = 0; k2 < 3; ++k2) nd[k2] = __builtin_fmaf(nb[k2] - na[k2], r.t, na[k2]);
+        uint32_t packed = 0;
+        for (int k2 = 0; k2 < 4; ++k2) {
+          const float mv = __builtin_fmaf(cb[k2] - ca[k2], r.t, ca[k2]);
+          const uint32_t u = !(mv > 0.0f) ? 0u : (mv >= 255.0f ? 255u : uint32_t(mv));
+          packed = (packed << 8) | u;
+        }
+        triCol[slot * 3 + v] = packed;
+      }
+    }
   }
 }
 
@@ -684,13 +704,13 @@ int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
   int rc = ensure(c, c->mcBlock, size_t(nbl) + 1, "mc block counts");
   if (rc) return rc;
   if ((rc = ensure(c, c->mcTotal, 4, "mc total"))) return rc;
-  if (c->keepCases && (rc = ensure(c, c->mcCases, size_t(g.cubes), "mc cases"))) return rc;
+  if ((rc = ensure(c, c->mcCases, std::max<size_t>(g.cubes, 1), "mc cases"))) return rc;
   if (nbl > 0) {
     rc = launch(c, "mc_classify_count", k_mc_count, dim3(unsigned(nbl)), dim3(kThreads), 0, g,
-                (const float*)c->field.p, m.isolevel, c->mcBlock.p,
-                c->keepCases ? c->mcCases.p : (uint8_t*)nullptr);
+                (const float*)c->field.p, m.isolevel, c->mcBlock.p, c->mcCases.p);
     if (rc) return rc;
   }
+  c->mcIso = m.isolevel;
   rc = launch(c, "mc_scan", k_mc_scan, dim3(1), dim3(1024), 0, c->mcBlock.p, uint32_t(nbl),
               c->mcTotal.p);
   if (rc) return rc;
@@ -700,6 +720,8 @@ int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
 
 int mcEmit(Ctx* c, const lf_mc_params& m) {
   if (!c->mcCounted) return fail(c, LF_ERR_STATE, "lf_mc_emit: no counts (call lf_mc_classify_count)");
+  if (!(m.isolevel == c->mcIso))
+    return fail(c, LF_ERR_ARG, "lf_mc_emit: isolevel differs from the one lf_mc_classify_count used");
   const CubeGrid g = cubeGrid(c);
   const uint64_t nbl = (uint64_t(g.cubes) + kCubesPerBlock - 1) / kCubesPerBlock;
   int rc = ensure(c, c->triPos, c->triCap * 9, "triangle positions");
@@ -721,7 +743,7 @@ int mcEmit(Ctx* c, const lf_mc_params& m) {
   A.attributes = attr;
   if (nbl > 0) {
     rc = launch(c, "mc_emit", k_mc_emit, dim3(unsigned(nbl)), dim3(kThreads), 0, A,
-                (const float*)c->field.p,
+                (const float*)c->field.p, (const uint8_t*)c->mcCases.p,
                 (const float4*)((attr && c->mcAttr) ? c->mcNormal.p : nullptr),
                 (const uint32_t*)((attr && c->mcAttr) ? c->mcColour.p : nullptr),
                 (const uint32_t*)c->mcBlock.p, (const uint32_t*)c->mcTotal.p, uint32_t(nbl),

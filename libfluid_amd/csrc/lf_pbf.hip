@@ -122,11 +122,13 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
 #pragma unroll
         for (int k = 0; k < kBuildBlock; ++k) {
           const uint32_t b = b0 + k;
-          if (!(b < e) || !(dist2(pa.x - pb[k].x, pa.y - pb[k].y, pa.z - pb[k].z) <= c.listR2))
-            continue;
+          const bool accept =
+              b < e && dist2(pa.x - pb[k].x, pa.y - pb[k].y, pa.z - pb[k].z) <= c.listR2;
+          // written whether accepted or not: slot cnt % 8 is the next free
+          // one, so a rejected candidate is overwritten by the next accepted
           win[(cnt % kNbrGroup) * BS + threadIdx.x] = b;
-          ++cnt;
-          if (cnt % kNbrGroup == 0u && cnt <= kMaxNbr) {
+          cnt += accept ? 1u : 0u;
+          if (accept && cnt % kNbrGroup == 0u && cnt <= kMaxNbr) {
             uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, cnt - kNbrGroup));
             g[0] = make_uint4(win[0 * BS + threadIdx.x], win[1 * BS + threadIdx.x],
                               win[2 * BS + threadIdx.x], win[3 * BS + threadIdx.x]);
