@@ -426,6 +426,10 @@ class Context:
     def profile(self, on=True):
         self._check(self._lib.lf_profile_enable(self._h, 1 if on else 0))
 
+    def profile_only(self, name=None):
+        """Time only the launches named `name` (None: every launch)."""
+        self._check(self._lib.lf_profile_only(self._h, name.encode() if name else None))
+
     def profile_reset(self):
         self._check(self._lib.lf_profile_reset(self._h))
 

@@ -185,6 +185,7 @@ SIGNATURES = {
     "lf_set_colliders": (_I, [_P, _SZ, _P]),
     "lf_wire_particles": (_I, [_P, _P, _SZ, ctypes.c_int64, ctypes.POINTER(_SZ)]),
     "lf_wire_triangles": (_I, [_P, _P, _SZ, ctypes.c_int64, ctypes.POINTER(_SZ)]),
+    "lf_profile_only": (_I, [_P, ctypes.c_char_p]),
     "lf_set_sources": (_I, [_P, _SZ, _P]),
     "lf_set_drains": (_I, [_P, _SZ, _P]),
     "lf_set_queries": (_I, [_P, _SZ, _P]),

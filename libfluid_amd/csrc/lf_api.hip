@@ -914,6 +914,12 @@ int lf_profile_enable(lf_ctx* c, int on) {
   return LF_OK;
 }
 
+int lf_profile_only(lf_ctx* c, const char* name) {
+  if (!c) return LF_ERR_ARG;
+  c->profOnly = name ? name : "";
+  return LF_OK;
+}
+
 int lf_profile_reset(lf_ctx* c) {
   if (!c) return LF_ERR_ARG;
   lf::drainTimings(c);

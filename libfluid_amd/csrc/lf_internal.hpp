@@ -518,6 +518,7 @@ struct Ctx {
 
   // profiling
   bool prof = false;
+  std::string profOnly;  // when set, only launches of this name are timed
   std::vector<PendingTiming> pending;
   std::vector<hipEvent_t> eventPool;
   std::map<std::string, KernelStat> stats;
@@ -591,11 +592,12 @@ template <typename... KArgs, typename... Args>
 int launch(Ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
            size_t shmem, Args... args) {
   PendingTiming t{};
-  if (c->prof) timingBegin(c, name, t);
+  const bool timed = c->prof && (c->profOnly.empty() || c->profOnly == name);
+  if (timed) timingBegin(c, name, t);
   hipLaunchKernelGGL(kernel, grid, block, shmem, c->stream, args...);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hipFail(c, e, name);
-  if (c->prof) timingEnd(c, t);
+  if (timed) timingEnd(c, t);
   return LF_OK;
 }
 
