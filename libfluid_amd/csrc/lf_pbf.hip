@@ -206,6 +206,11 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
   if (!usable && cnt <= kMaxNbr) usable = listUsableLocal(c, L, a, pa.x, pa.y, pa.z, key);
   const uint64_t fell = __ballot(!usable);
   if (fell && __lane_id() == __builtin_ctzll(fell)) atomicAdd(&L.maxDisp[pass], uint32_t(__popcll(fell)));
+  // A wave with any lane walking its cells walks cells on every lane: the
+  // walk takes as long for one lane as for 64, and list + walk in sequence
+  // cost more than the walk alone. The accepted neighbours and their order
+  // are the same either way.
+  if (fell) usable = false;
   if (usable) {
     static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
     for (uint32_t i = 0; i < cnt; i += BLK) {
