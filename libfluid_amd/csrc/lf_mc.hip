@@ -72,10 +72,11 @@ constexpr uint32_t kUnroll = LF_MC_UNROLL;  // staged particles in flight per sa
 constexpr int kHaloCells = 512;
 constexpr int kStageCap = 2048;           // staged particles (32 KB)
 
+// Outside h (and for a NaN or infinite d2) t is clamped to 0, so the term is
+// exactly 0, as the test d2 <= h^2 would make it.
 __device__ inline float densityTerm(const PbfConsts& c, float4 pb, float px, float py, float pz) {
   const float d2 = dist2(px - pb.x, py - pb.y, pz - pb.z);
-  if (!(d2 <= c.hh)) return 0.0f;
-  const float t = c.hh - d2;
+  const float t = fmaxf(c.hh - d2, 0.0f);
   return pb.w * (c.poly6 * ((t * t) * t));
 }
 
