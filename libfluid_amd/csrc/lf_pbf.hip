@@ -32,15 +32,17 @@ namespace lf {
 namespace {
 
 #ifndef LF_BLOCK_LAMBDA
-#define LF_BLOCK_LAMBDA 8
+#define LF_BLOCK_LAMBDA 16
 #endif
 #ifndef LF_BLOCK_DELTA
-#define LF_BLOCK_DELTA 8
+#define LF_BLOCK_DELTA 16
 #endif
 #ifndef LF_BLOCK_XSPH
-#define LF_BLOCK_XSPH 8
+#define LF_BLOCK_XSPH 16
 #endif
-constexpr int kBlockLambda = LF_BLOCK_LAMBDA;  // list entries in flight per lane (forEachAccepted)
+// list entries in flight per lane (forEachAccepted); 16 measured against 8 at
+// 1M: XSPH 211 -> 193 us, FAST_MATH lambda 104 -> 93 us, the combined batch -9 %
+constexpr int kBlockLambda = LF_BLOCK_LAMBDA;
 constexpr int kBlockDelta = LF_BLOCK_DELTA;
 constexpr int kBlockXsph = LF_BLOCK_XSPH;
 #ifndef LF_BLOCK_BUILD
@@ -213,6 +215,7 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
   if (fell) usable = false;
   if (usable) {
     static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
+    static_assert(kMaxNbr % BLK == 0, "a block never reads past the list capacity");
     for (uint32_t i = 0; i < cnt; i += BLK) {
       uint32_t b[BLK];
 #pragma unroll
