@@ -269,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_drain_scatter(
 // count and the mean of their ARGB bytes packed back (truncated), from the
 // colours the step started with.
 __global__ __launch_bounds__(kThreads) void k_queries(uint32_t nq, const float* __restrict__ pts,
-                                                      PbfConsts c,
+                                                      uint32_t n, PbfConsts c,
                                                       const uint32_t* __restrict__ table,
                                                       const uint8_t* __restrict__ T,
                                                       const float4* __restrict__ V,
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kThreads) void k_queries(uint32_t nq, const float* 
                                     cellCoord((pts[3 * q + 1] / c.scale - c.minExt[1]) / c.h),
                                     cellCoord((pts[3 * q + 2] / c.scale - c.minExt[2]) / c.h));
   uint32_t N = 0, packed = 0;
-  if (key + 1u < c.G) {
+  if (n > 0 && key + 1u < c.G) {  // with no particles the table was not rebuilt
     float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     const uint32_t e = table[key + 1u];
     for (uint32_t a = table[key]; a < e; ++a) {
@@ -338,7 +338,8 @@ int answerQueries(Ctx* c) {
   int rc = ensure(c, c->queryOut, 2 * size_t(c->nQueries), "query results");
   if (rc) return rc;
   rc = launch(c, "queries", k_queries, dim3(blocksFor(c->nQueries, kThreads)), dim3(kThreads), 0,
-              c->nQueries, (const float*)c->queryPts.p, c->consts, (const uint32_t*)c->table.p,
+              c->nQueries, (const float*)c->queryPts.p, uint32_t(c->n), c->consts,
+              (const uint32_t*)c->table.p,
               (const uint8_t*)c->T[c->cur].p, (const float4*)c->V[c->cur].p, c->queryOut.p);
   if (rc) return rc;
   c->queriesAnswered = true;

@@ -121,3 +121,30 @@ def test_query_results_need_a_step():
         c.step(p)
         counts, colours = c.query_results()
         assert counts.shape == (2,)
+
+
+def test_drains_empty_the_scene_and_sources_refill():
+    """A drain that takes every particle leaves an empty state whose step and
+    queries still work (the reference's depleted case, oclsph.hpp:711-715);
+    a source then refills it."""
+    sc = scenes.small_block(n_side=6)
+    p = sc.params()
+    with lf.Context(0) as c:
+        c.upload(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
+        c.set_drains([lf.Drain(centre=(0.5, 0.5, 0.5), width=10.0, depth=0.0)])
+        c.set_queries(QUERIES[:2])
+        c.step(p)
+        assert c.n == 0
+        counts, colours = c.query_results()
+        assert (counts == 0).all() and (colours == 0).all()
+        c.step(p)  # an empty step
+        assert c.n == 0
+        c.set_drains([])
+        c.set_sources([lf.Source(centre=(0.5, 0.5, 0.5), velocity=(0.0, 0.0, 0.0), rate=16,
+                                 tag=1, colour=0xFFFFFFFF)])
+        c.step(p)
+        assert c.n == 16
+        d = c.download()
+        assert np.isfinite(d["position"]).all() and (d["colour"] == 0xFFFFFFFF).all()
+        counts, _ = c.query_results()
+        assert counts.shape == (2,)
