@@ -49,6 +49,10 @@ constexpr int kBlockXsph = LF_BLOCK_XSPH;
 #define LF_BLOCK_BUILD 4
 #endif
 constexpr int kBuildBlock = LF_BLOCK_BUILD;  // candidates loaded together (list build)
+// lists are padded (with the particle's own index) to whole blocks of this
+// many entries, the largest block a walk reads
+constexpr uint32_t kListPad = 16;
+static_assert(kMaxNbr % kListPad == 0, "padding stays within the list capacity");
 
 
 struct Lists {
@@ -140,11 +144,25 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
         }
       }
     });
-    // the last cnt % 8 entries
-    const uint32_t r = cnt % kNbrGroup;
-    if (r != 0u && cnt <= kMaxNbr) {
-      uint32_t* g = L.nbr + nbrSlot(a, cnt - r);
-      for (uint32_t j = 0; j < r; ++j) g[j] = win[j * BS + threadIdx.x];
+    // The last cnt % 8 entries, and the list padded with a itself to whole
+    // blocks of kListPad entries: readers load whole blocks and skip the
+    // entries past the count, so every loaded index is a valid particle.
+    if (cnt <= kMaxNbr) {
+      const uint32_t r = cnt % kNbrGroup;
+      if (r != 0u) {
+        uint32_t e[kNbrGroup];
+#pragma unroll
+        for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = j < r ? win[j * BS + threadIdx.x] : a;
+        uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, cnt - r));
+        g[0] = make_uint4(e[0], e[1], e[2], e[3]);
+        g[1] = make_uint4(e[4], e[5], e[6], e[7]);
+      }
+      for (uint32_t s = (cnt + kNbrGroup - 1) / kNbrGroup * kNbrGroup;
+           s % kListPad != 0u; s += kNbrGroup) {
+        uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, s));
+        g[0] = make_uint4(a, a, a, a);
+        g[1] = make_uint4(a, a, a, a);
+      }
     }
     L.count[a] = cnt;
   });
@@ -196,8 +214,8 @@ __device__ inline const float4& posOf(const PosVel& r) { return r.p; }
 // load(b) returns the candidate's float4 (xyz = p*). The list is walked in
 // blocks of BLK entries: BLK index loads, then BLK gathers, then the BLK pair
 // terms in list order, so a block costs one memory latency instead of BLK.
-// Entries past the count (the tail of the last block) are replaced by a itself
-// (a valid address) and skipped.
+// Entries past the count (the tail of the last block) hold a itself (the
+// build pads every list to whole blocks), a valid address, and are skipped.
 template <int BLK, typename Load, typename Use>
 __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int pass, bool fresh,
                                        uint32_t a, float4 pa, uint32_t key,
@@ -215,17 +233,16 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
   if (fell) usable = false;
   if (usable) {
     static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
-    static_assert(kMaxNbr % BLK == 0, "a block never reads past the list capacity");
+    static_assert(kListPad % BLK == 0, "the build pads lists to whole blocks");
     for (uint32_t i = 0; i < cnt; i += BLK) {
       uint32_t b[BLK];
 #pragma unroll
-      for (int k = 0; k < BLK; k += 4) {  // 16-byte loads; entries past cnt read as a
+      for (int k = 0; k < BLK; k += 4) {  // 16-byte loads; entries past cnt hold a
         const uint4 q = *reinterpret_cast<const uint4*>(L.nbr + nbrSlot(a, i + k));
         b[k] = q.x;
-        b[k + 1] = (i + k + 1 < cnt) ? q.y : a;
-        b[k + 2] = (i + k + 2 < cnt) ? q.z : a;
-        b[k + 3] = (i + k + 3 < cnt) ? q.w : a;
-        if (!(i + k < cnt)) b[k] = a;
+        b[k + 1] = q.y;
+        b[k + 2] = q.z;
+        b[k + 3] = q.w;
       }
       decltype(load(0u)) p[BLK];
 #pragma unroll
