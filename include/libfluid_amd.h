@@ -165,7 +165,9 @@ int lf_sync(lf_ctx* ctx);
 
 /* ---- particle state (device resident) ----------------------------------- */
 /* Upload n particles (SoA, vec3 packed as 3 floats). Replaces the SoA pack and
- * upload of oclsph.hpp:822-853. The upload order is the "caller order". */
+ * upload of oclsph.hpp:822-853. The upload order is the "caller order".
+ * n < 2^28 per context (kernels address particles with 32-bit byte offsets;
+ * 2^28 particles need about 240 GB of device memory). */
 int lf_state_upload(lf_ctx* ctx, size_t n, const float* position,
                     const float* velocity, const float* mass,
                     const int32_t* type, const uint32_t* colour);

@@ -276,7 +276,7 @@ static int appendSources(Ctx* c) {
   }
   const size_t k = X.size(), n = c->n;
   if (k == 0) return LF_OK;
-  if (n + k >= (size_t(1) << 31)) return fail(c, LF_ERR_ARG, "sources: too many particles");
+  if (n + k >= (size_t(1) << 28)) return fail(c, LF_ERR_ARG, "sources: at most 2^28 - 1 particles per context");
   const int cur = c->cur;
   int rc;
   if ((rc = ensureKeep(c, c->X[cur], n + k, n, "positions"))) return rc;
@@ -439,7 +439,8 @@ int lf_state_upload(lf_ctx* c, size_t n, const float* pos, const float* vel, con
   if (!c) return LF_ERR_ARG;
   if (n > 0 && (!pos || !vel || !mass || !type || !colour))
     return fail(c, LF_ERR_ARG, "lf_state_upload: null array");
-  if (n >= (size_t(1) << 31)) return fail(c, LF_ERR_ARG, "lf_state_upload: n too large");
+  if (n >= (size_t(1) << 28))  // gathers address particles with 32-bit byte offsets
+    return fail(c, LF_ERR_ARG, "lf_state_upload: at most 2^28 - 1 particles per context");
   LF_HIP(c, hipSetDevice(c->device));
   int rc;
   const int cur = c->cur;

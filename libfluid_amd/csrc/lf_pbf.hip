@@ -101,8 +101,8 @@ __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint3
 
 // Neighbour lists of the predicted p* (see the file comment). P0 (the
 // clamped p*) was written by the reorder kernel. Each lane collects its
-// accepted candidates in an 8-entry register window and stores it as one
-// 32-byte group of its list whenever it fills.
+// accepted candidates in an 8-entry LDS window and stores it as one 32-byte
+// group of its list whenever it fills.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ table,
@@ -209,6 +209,14 @@ struct PosVel {
 __device__ inline const float4& posOf(const float4& r) { return r; }
 __device__ inline const float4& posOf(const PosVel& r) { return r.p; }
 
+// base[i] addressed as base + a 32-bit byte offset (the base stays in scalar
+// registers, no 64-bit address arithmetic per gather). Particle indices are
+// below 2^28 (checked at upload), so i * 16 fits.
+template <typename T> __device__ inline T at(const T* __restrict__ base, uint32_t i) {
+  static_assert(sizeof(T) <= 16, "records of at most 16 bytes");
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(T)));
+}
+
 // Visits the accepted neighbours (r <= h) of particle a at position pa, in
 // the reference's order: through its list when usable, else the 27 cells.
 // load(b) returns the candidate's float4 (xyz = p*). The list is walked in
@@ -287,7 +295,7 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
     }
     LambdaSums A;
     forEachAccepted<kBlockLambda>(
-        c, L, 1, fresh != 0, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return P[b]; },
+        c, L, 1, fresh != 0, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return at(P, b); },
         [&](uint32_t, float4 pb, float d2) {
           lambdaTerm<KAHAN, FAST>(c, pa.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, A);
         });
@@ -345,7 +353,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         forEachAccepted<kBlockDelta>(
             c, L, 2, fresh != 0, a, pa, keys[a], table + c.keyBase,
-            [&](uint32_t b) { return PL[b]; },
+            [&](uint32_t b) { return at(PL, b); },
             [&](uint32_t, float4 pb, float d2) {
               deltaTerm<FAST>(c, la, pb.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, ax, ay, az);
             });
@@ -514,7 +522,7 @@ __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict_
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     forEachAccepted<kBlockXsph>(
         c, L, 3, false, a, pa, keys[a], table + c.keyBase,
-        [&](uint32_t b) { return PosVel{P[b], Vin[b]}; },
+        [&](uint32_t b) { return PosVel{at(P, b), at(Vin, b)}; },
         [&](uint32_t, const PosVel& r, float d2) {
           const float4& pb = r.p;
           // P.w is the fluid mass, 0 for obstacles: they add zero-weight terms.
