@@ -92,9 +92,10 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
                                                          const float4* __restrict__ P,
                                                          float* __restrict__ field,
                                                          uint32_t tilesY, uint32_t tilesZ) {
-  // staged particles as separate x, y, z, m arrays: lanes reading different
-  // particles then hit different banks far more often than with 16-byte records
-  __shared__ float stX[kStageCap], stY[kStageCap], stZ[kStageCap], stM[kStageCap];
+  // staged particles as 16-byte (x, y, z, m) records: one LDS read per
+  // candidate at a constant offset from one address (separate x/y/z/m arrays
+  // 8 KB apart need extra address arithmetic)
+  __shared__ float4 staged[kStageCap];
   __shared__ uint32_t cellStart[kHaloCells + 1];
   __shared__ uint32_t scanTmp[kThreads / 64];
   __shared__ int mode;  // 0 LDS, 1 global
@@ -217,10 +218,7 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
 #pragma unroll
           for (uint32_t u = 0; u < 4; ++u)
             if (j + u < cnt) {
-              stX[dst + j + u] = q[u].x;
-              stY[dst + j + u] = q[u].y;
-              stZ[dst + j + u] = q[u].z;
-              stM[dst + j + u] = q[u].w;
+              staged[dst + j + u] = q[u];
             }
         }
       }
@@ -256,13 +254,13 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
           float t[kUnroll];
 #pragma unroll
           for (uint32_t u = 0; u < kUnroll; ++u)
-            t[u] = densityTerm(c, make_float4(stX[b + u], stY[b + u], stZ[b + u], stM[b + u]),
+            t[u] = densityTerm(c, staged[b + u],
                                px, py, pz);
 #pragma unroll
           for (uint32_t u = 0; u < kUnroll; ++u) rho += t[u];
         }
         for (; b < e; ++b)
-          rho += densityTerm(c, make_float4(stX[b], stY[b], stZ[b], stM[b]), px, py, pz);
+          rho += densityTerm(c, staged[b], px, py, pz);
       }
   } else {
     forEachNeighbourCell(cellKey(c, px, py, pz), table, c.G, [&](uint32_t s, uint32_t e) {
