@@ -20,6 +20,7 @@ of the same scene.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -45,6 +46,8 @@ def algorithmic_bytes(name, n, G, S, T):
         "pbf_lambda": 21 * n + 4 * n + 4 * G + 4 * n,  # p*, m, type, key, table -> lambda, rho
         "pbf_lambda_kahan": 21 * n + 4 * n + 4 * G + 4 * n,
         "pbf_delta": 29 * n + 4 * n + 4 * G,   # p*, lambda, type, key, mass, table -> p*
+        # the last delta pass of a step, finalise fused: + x, v in and out
+        "pbf_delta_final": 29 * n + 4 * n + 4 * G + 48 * n,
         "pbf_finalise": 1 * n + 12 * n + 24 * n + 24 * n,  # type, p*, x, v -> x, v
         "pbf_xsph": 12 * n + 1 * n + 12 * n + 4 * n + 4 * G + 12 * n,
         "pbf_diffuse": 1 * n + 4 * n + 4 * n + 4 * G + 4 * n,
@@ -55,12 +58,14 @@ def algorithmic_bytes(name, n, G, S, T):
     return table.get(name)
 
 
-# The kernel (rocprof name prefix) behind each launch name of the library.
+# The kernel behind each launch name of the library: a rocprof name prefix, or
+# a pattern of the full name where template arguments tell launches apart.
 KERNEL_SYMBOL = {
     "predict_key": "k_predict_key", "radix_hist": "k_radix_hist", "radix_scan": "k_radix_scan",
     "radix_scatter": "k_radix_scatter", "reorder": "k_reorder", "cell_table": "k_cell_table",
     "pbf_build_list": "k_build_list", "pbf_lambda": "k_lambda", "pbf_lambda_kahan": "k_lambda",
-    "pbf_delta": "k_delta", "pbf_finalise": "k_finalise", "pbf_xsph": "k_xsph",
+    "pbf_delta": r"k_delta<\d+, false, false>", "pbf_delta_final": r"k_delta<\d+, false, true>",
+    "pbf_finalise": "k_finalise", "pbf_xsph": "k_xsph",
     "pbf_diffuse": "k_diffuse", "mc_density_field": "k_mc_density", "mc_classify_count": "k_mc_count",
     "mc_scan": "k_mc_scan", "mc_emit": "k_mc_emit", "copy_probe": "k_copy_probe",
 }
@@ -342,7 +347,12 @@ def main():
 
     def traffic_of(name):
         sym = KERNEL_SYMBOL.get(name)
-        hits = [v for k, v in traffic.items() if sym and (k == sym or k.startswith(sym + "<"))]
+        if not sym:
+            return None
+        if "<" in sym:
+            hits = [v for k, v in traffic.items() if re.fullmatch(sym, k)]
+        else:
+            hits = [v for k, v in traffic.items() if k == sym or k.startswith(sym + "<")]
         return hits[0] if hits else None
 
     value = n * args.steps / elapsed

@@ -305,12 +305,16 @@ static int pbfStepOnce(Ctx* c) {
   if ((rc = pbfBuildLists(c))) return rc;
   if (c->consts.flags & LF_FLAG_DIFFUSE)
     if ((rc = pbfDiffuse(c))) return rc;
-  for (uint32_t it = 0; it < c->params.iterations; ++it) {
+  // without the convergence check every pass runs, so the last delta pass can
+  // finalise each particle itself
+  const uint32_t iters = c->params.iterations;
+  const bool fuse = !c->converge && iters > 0;
+  for (uint32_t it = 0; it < iters; ++it) {
     if ((rc = pbfLambda(c, it > 0))) return rc;
     if (c->converge && (rc = densityCheck(c, int(it)))) return rc;
-    if ((rc = pbfDelta(c, true))) return rc;
+    if ((rc = pbfDelta(c, true, fuse && it + 1 == iters))) return rc;
   }
-  if ((rc = pbfFinalise(c))) return rc;
+  if (!fuse && (rc = pbfFinalise(c))) return rc;
   if ((rc = pbfXsph(c))) return rc;
   c->stepDone = true;
   return LF_OK;

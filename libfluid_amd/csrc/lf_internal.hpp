@@ -635,7 +635,7 @@ int gridBuild(Ctx* c);                        // lf_grid.hip
 int radixSort(Ctx* c, size_t n, int& outIdx);  // lf_sort.hip
 int pbfBuildLists(Ctx* c);                    // lf_pbf.hip
 int pbfLambda(Ctx* c, bool skipDone);  // skipDone: skip the scenes that converged
-int pbfDelta(Ctx* c, bool skipDone);
+int pbfDelta(Ctx* c, bool skipDone, bool final = false);  // final: finalise fused in
 int pbfFinalise(Ctx* c);
 int pbfXsph(Ctx* c);
 int pbfDiffuse(Ctx* c);

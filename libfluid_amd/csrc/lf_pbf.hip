@@ -330,14 +330,20 @@ __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float d
 // writes Pout. deltaP is not stored (nothing reads it after the pass). Also
 // raises D and the per-cell displacement to the new positions' (one atomic
 // per wave, and per run of equal keys).
-template <int BS, bool FAST>
+//
+// FINAL (the last pass of a step whose passes all run): also does
+// k_finalise's work for the particle, from the same values in the same
+// order, so the separate finalise launch and its pass over the state go.
+template <int BS, bool FAST, bool FINAL>
 __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict__ keys,
                                               const uint32_t* __restrict__ table,
                                               const uint8_t* __restrict__ T,
                                               const float4* __restrict__ PL,
-                                              const float4* __restrict__ X, Lists L, int fresh,
+                                              float4* __restrict__ X, Lists L, int fresh,
                                               float4* __restrict__ Pout, float4* __restrict__ V,
-                                              ColliderGrid mesh) {
+                                              ColliderGrid mesh,
+                                              const uint32_t* __restrict__ diffused,
+                                              int useDiffused) {
   __shared__ float waveMax[BS / 64];
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
@@ -347,7 +353,8 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
       const float4 pa = PL[a];
       const float mass = X[a].w;
       if (T[a] == LF_OBSTACLE) {
-        Pout[a] = make_float4(pa.x, pa.y, pa.z, mass);
+        // finalise leaves obstacles in place with P.w = 0 (no fluid mass)
+        Pout[a] = make_float4(pa.x, pa.y, pa.z, FINAL ? 0.0f : mass);
       } else {
         const float la = pa.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
@@ -378,6 +385,18 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
           o[k] = q / c.scale;
         }
         Pout[a] = make_float4(o[0], o[1], o[2], mass);
+        if (FINAL) {  // k_finalise on p* = o; V[a] after the collider response
+          const float4 x = X[a];
+          const float4 v = V[a];
+          const float dX = o[0] - x.x / c.scale;
+          const float dY = o[1] - x.y / c.scale;
+          const float dZ = o[2] - x.z / c.scale;
+          X[a] = make_float4(o[0] * c.scale, o[1] * c.scale, o[2] * c.scale, x.w);
+          const float col = useDiffused ? __uint_as_float(diffused[a]) : v.w;
+          V[a] = make_float4(__builtin_fmaf(dX, c.invDt, v.x) * c.vd,
+                             __builtin_fmaf(dY, c.invDt, v.y) * c.vd,
+                             __builtin_fmaf(dZ, c.invDt, v.z) * c.vd, col);
+        }
         const float4 p0 = L.P0[a];
         disp = __builtin_sqrtf(dist2(o[0] - p0.x, o[1] - p0.y, o[2] - p0.z));
         if (!(disp < 3.0e38f)) disp = 3.0e38f;  // NaN / inf: every list unusable
@@ -590,20 +609,26 @@ int pbfLambda(Ctx* c, bool skipDone) {
   });
 }
 
-int pbfDelta(Ctx* c, bool skipDone) {
+int pbfDelta(Ctx* c, bool skipDone, bool final) {
   if (c->n == 0) return LF_OK;
   int grc = prepareColliderGrid(c);
   if (grc) return grc;
   const Work W = workOf(c, kSlotDelta, skipDone);
   const bool fastMath = (c->consts.flags & LF_FLAG_FAST_MATH) != 0;
+  const char* name = final ? (fastMath ? "pbf_delta_final_fast" : "pbf_delta_final")
+                           : (fastMath ? "pbf_delta_fast" : "pbf_delta");
   const int rc = withBS(c, [&](auto bs) {
     return withFlag(fastMath, [&](auto fast) {
-      return launch(c, fastMath ? "pbf_delta_fast" : "pbf_delta",
-                    k_delta<decltype(bs)::value, decltype(fast)::value>, gridOf(c, W),
-                    dim3(decltype(bs)::value), 0, W, (const uint32_t*)c->localKey.p,
-                    (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
-                    (const float4*)c->PL.p, (const float4*)c->X[c->cur].p, listsOf(c),
-                    int(c->listFresh), c->P.p, c->V[c->cur].p, c->colliderGrid);
+      return withFlag(final, [&](auto fin) {
+        return launch(c, name,
+                      k_delta<decltype(bs)::value, decltype(fast)::value, decltype(fin)::value>,
+                      gridOf(c, W), dim3(decltype(bs)::value), 0, W,
+                      (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
+                      (const uint8_t*)c->T[c->cur].p, (const float4*)c->PL.p, c->X[c->cur].p,
+                      listsOf(c), int(c->listFresh), c->P.p, c->V[c->cur].p, c->colliderGrid,
+                      (const uint32_t*)c->diffused.p,
+                      int((c->consts.flags & LF_FLAG_DIFFUSE) != 0));
+      });
     });
   });
   c->listFresh = false;  // p* now differs from the predicted positions
