@@ -303,19 +303,27 @@ def main():
         ctx.mc_classify_count(mc)
         ctx.mc_emit(p, mc)
 
-    # warm-up with every launch timed: names the kernel with the largest share
-    ctx.profile(True)
-    ctx.profile_reset()
+    # pass 1: warm-up, then the K steps with every launch timed: the per-kernel
+    # table, the device time per step, and the kernel with the largest total
     for _ in range(args.warmup):
         one_step()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(args.steps):
+        one_step()
     ctx.sync()
-    warm = ctx.kernel_stats()
-    dom_name = max(warm.items(), key=lambda kv: kv[1][1])[0] if warm else None
-    print("bench: warmup done, dominant kernel %s" % dom_name, file=sys.stderr, flush=True)
+    stats = ctx.kernel_stats()
+    dom_name = max(stats.items(), key=lambda kv: kv[1][1])[0] if stats else None
+    print("bench: profiled pass done, dominant kernel %s" % dom_name, file=sys.stderr, flush=True)
 
-    # timed region: HIP events around the dominant kernel's launches only (the
-    # events of every launch cost ~8 % of a step)
+    # pass 2, timed: the same steps from the same start, HIP events around the
+    # dominant kernel's launches only (events on every launch cost ~8 % of a step)
+    ctx.profile(False)
+    ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
+    for _ in range(args.warmup):
+        one_step()
     ctx.profile_only(dom_name)
+    ctx.profile(True)
     ctx.profile_reset()
     ctx.sync()
     t0 = time.perf_counter()
@@ -326,20 +334,8 @@ def main():
     print("bench: %d timed steps in %.3f s" % (args.steps, elapsed), file=sys.stderr, flush=True)
     dom_stats = ctx.kernel_stats()
     ntri = ctx.mc_count()
-
-    # the same steps again from the same start, every launch timed: the
-    # per-kernel table and the device time per step
     ctx.profile_only(None)
     ctx.profile(False)
-    ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
-    for _ in range(args.warmup):
-        one_step()
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(args.steps):
-        one_step()
-    ctx.sync()
-    stats = ctx.kernel_stats()
     G = ctx.grid_info()[0]
     S = int(np.prod(mc.samples))
 
@@ -398,7 +394,7 @@ def main():
                      "traffic": traffic_of(dom_name), "alg_bytes_per_launch": dom_bytes,
                      "avg_launch_us": round(dom_avg_s * 1e6, 2)},
         "kernels": kernels,
-        "kernels_pass": "the timed steps repeated from the same start with every launch timed",
+        "kernels_pass": "the timed steps run beforehand from the same start with every launch timed",
         "device_ms_per_step": device_ms / args.steps,
     }
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731

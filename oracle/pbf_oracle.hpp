@@ -614,8 +614,9 @@ template <typename F> class Pbf {
         }
   }
 
-  /* SPH density field rho(x) = sum over fluid b of m_b poly6 (h^2 - d^2)^3,
-   * d^2 = |x - p*_b|^2 <= h^2 (W6 written on d^2), on the lattice
+  /* SPH density field rho(x) = sum over fluid b of m_b poly6 (h^2 - d^2)^3 for
+   * |x - p*_b| < h, evaluated as (m_b poly6) t^3 with t = h^2 - dz^2 - dy^2 - dx^2
+   * taken as one fused multiply-add chain from h^2, on the lattice
    * x_ijk = origin + (i,j,k) * step (solver units). Candidates: the 27 cells
    * around the sample's own cell, as the PBF passes; terms added in order. */
   void mcDensityField(const uint32_t S[3], const F origin[3], const F step[3],
@@ -633,10 +634,10 @@ template <typename F> class Pbf {
           forEachCandidate(keyOf(p), [&](uint32_t b) {
             if (ts[b] == LF_OBSTACLE) return;
             const F* pb = &ps[4 * b];
-            const F d2 = dist2(p[0] - pb[0], p[1] - pb[1], p[2] - pb[2]);
-            if (!(d2 <= c.hh)) return;
-            const F t = c.hh - d2;
-            r += pb[3] * (c.poly6 * ((t * t) * t));
+            const F dx = p[0] - pb[0], dy = p[1] - pb[1], dz = p[2] - pb[2];
+            const F t = std::fma(-dx, dx, std::fma(-dy, dy, std::fma(-dz, dz, c.hh)));
+            if (!(t > F(0))) return;
+            r += (pb[3] * c.poly6) * ((t * t) * t);
           });
           field[(size_t(xl) * S[1] + size_t(yl)) * S[2] + z] = r;
         }

@@ -327,6 +327,39 @@ def test_fp32_oracle_close_to_fp64():
     assert np.max(np.abs(r32 - r64) / r64) < 1e-5
 
 
+def test_density_field_matches_fp64_brute_force():
+    """The benchmark density field rho(x) = sum_b m_b 315/(64 pi h^9) (h^2 - d^2)^3
+    over |x - p_b| < h, from an FP64 sum over every particle (no cells, KD-tree
+    radius search) on a non-cubic lattice: pins the lattice geometry and axis
+    order, the poly6 constant, the cube, the masses and the 27-cell candidate
+    set of both oracle precisions."""
+    from scipy.spatial import cKDTree
+    sc = scenes.dam_break_8k()
+    p = sc.params()
+    S = (48, 24, 20)
+    h = float(np.float32(scenes.SOLVER_H))  # the parameter block holds h as FP32
+    poly6 = 315.0 / (64.0 * math.pi * h ** 9)
+    for fp64, rel in ((True, 1e-11), (False, 1e-5)):
+        o = oracle.Pbf(p, fp64=fp64)
+        o.load(sc.position, sc.velocity, sc.mass)
+        o.grid_build()
+        field = o.mc_density(S)
+        ps = o.get(LF_ARR_PSTAR)
+        assert np.array_equal(ps[:, 3], np.ones(sc.n))
+        origin, step = o.density_lattice(S)
+        ax = [origin[k] + np.arange(S[k]) * step[k] for k in range(3)]
+        X = np.stack(np.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3)
+        want = np.zeros(len(X))
+        tree = cKDTree(ps[:, :3])
+        for i, nb in enumerate(tree.query_ball_point(X, h)):
+            if nb:
+                d2 = ((ps[nb, :3] - X[i]) ** 2).sum(1)
+                want[i] = (ps[nb, 3] * poly6 * np.maximum(h * h - d2, 0.0) ** 3).sum()
+        want = want.reshape(S)
+        assert (want > 0).sum() > 2000
+        assert np.max(np.abs(field - want)) <= rel * want.max(), fp64
+
+
 # ---- marching cubes ---------------------------------------------------------------
 
 def test_mc_cases_are_thresholds():
