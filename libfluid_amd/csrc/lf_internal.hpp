@@ -135,7 +135,7 @@ constexpr uint32_t kMaxNbr = 128;
 // contiguous 2 KB run (two 16-byte loads per lane); the build stores a full
 // 32-byte group per lane at a time.
 constexpr uint32_t kNbrGroup = 8;
-__host__ __device__ inline uint64_t nbrSlot(uint32_t a, uint32_t i) {
+__host__ __device__ constexpr uint64_t nbrSlot(uint32_t a, uint32_t i) {
   return ((uint64_t(a >> 6) * (kMaxNbr / kNbrGroup) + (i / kNbrGroup)) << 9) |
          (uint64_t(a & 63u) << 3) | (i % kNbrGroup);
 }

@@ -99,17 +99,31 @@ __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint3
   return da + __uint_as_float(m) < c.skinSafe;
 }
 
+// base[i] addressed as base + a 32-bit byte offset (the base stays in scalar
+// registers, no 64-bit address arithmetic per gather). Particle indices are
+// below 2^28 (checked at upload), so i * 16 fits.
+template <typename T> __device__ inline T at(const T* __restrict__ base, uint32_t i) {
+  static_assert(sizeof(T) <= 16, "records of at most 16 bytes");
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(T)));
+}
+
 // Neighbour lists of the predicted p* (see the file comment). P0 (the
 // clamped p*) was written by the reorder kernel. Each lane collects its
-// accepted candidates in an 8-entry LDS window and stores it as one 32-byte
-// group of its list whenever it fills.
+// accepted candidates in a 16-entry LDS ring (two 8-entry groups) and, once
+// per block of candidates, stores a completed group as 32 bytes of its list:
+// a block adds fewer than 8 entries, so at most one group completes in it and
+// the group still to be stored is never overwritten. (Checking after every
+// candidate ran the store path on almost every step: with 64 lanes, some lane
+// nearly always completes a group.)
 template <int BS>
 __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ table,
                                                    const uint8_t* __restrict__ T, Lists L) {
-  // per lane, the entries of the group being filled: win[j * BS + lane] holds
-  // entry (group start + j); lanes side by side, so no bank conflicts
-  __shared__ uint32_t win[kNbrGroup * BS];
+  constexpr uint32_t kRing = 2 * kNbrGroup;
+  static_assert(kBuildBlock < int(kNbrGroup), "a block completes at most one group");
+  // win[j * BS + lane] holds entry j mod 16 of the lane's list; lanes side by
+  // side, so no bank conflicts
+  __shared__ uint32_t win[kRing * BS];
   if (blockIdx.x == 0 && threadIdx.x < 4) L.maxDisp[threadIdx.x] = 0u;
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
@@ -119,28 +133,41 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
       return;
     }
     const float4 pa = L.P0[a];
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, stored = 0;  // entries accepted / written to the list (a multiple of 8)
+    // group i / 8 of a's list starts at nbrSlot(a, 0) + (i / 8) * 512 entries
+    char* const list0 = reinterpret_cast<char*>(L.nbr + nbrSlot(a, 0));
+    static_assert(nbrSlot(0, kNbrGroup) == 512u, "groups of a particle 512 entries apart");
+    auto storeGroup = [&](uint32_t first, uint32_t valid) {  // entries [first, first + 8)
+      // first is a multiple of 8: its slots are (first & 8) + 0..7, no wrap
+      const uint32_t* w = win + (first & kNbrGroup) * BS + threadIdx.x;
+      uint32_t e[kNbrGroup];
+#pragma unroll
+      for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = j < valid ? w[j * BS] : a;
+      uint4* g = reinterpret_cast<uint4*>(list0 + (first / kNbrGroup) * (512u * 4u));
+      g[0] = make_uint4(e[0], e[1], e[2], e[3]);
+      g[1] = make_uint4(e[4], e[5], e[6], e[7]);
+    };
     forEachNeighbourCellRolled(keys[a], table + c.keyBase, c.G, [&](uint32_t s, uint32_t e) {
       for (uint32_t b0 = s; b0 < e; b0 += kBuildBlock) {
         float4 pb[kBuildBlock];  // candidates loaded together (indices clamped into the cell)
 #pragma unroll
-        for (int k = 0; k < kBuildBlock; ++k) pb[k] = L.P0[min(b0 + k, e - 1u)];
+        for (int k = 0; k < kBuildBlock; ++k) pb[k] = at(L.P0, min(b0 + k, e - 1u));
 #pragma unroll
         for (int k = 0; k < kBuildBlock; ++k) {
           const uint32_t b = b0 + k;
-          const bool accept =
-              b < e && dist2(pa.x - pb[k].x, pa.y - pb[k].y, pa.z - pb[k].z) <= c.listR2;
-          // written whether accepted or not: slot cnt % 8 is the next free
+          // both tests always evaluated (the load index is clamped into the
+          // cell): no branch per candidate
+          const uint32_t accept =
+              (b < e ? 1u : 0u) &
+              (dist2(pa.x - pb[k].x, pa.y - pb[k].y, pa.z - pb[k].z) <= c.listR2 ? 1u : 0u);
+          // written whether accepted or not: slot cnt % 16 is the next free
           // one, so a rejected candidate is overwritten by the next accepted
-          win[(cnt % kNbrGroup) * BS + threadIdx.x] = b;
-          cnt += accept ? 1u : 0u;
-          if (accept && cnt % kNbrGroup == 0u && cnt <= kMaxNbr) {
-            uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, cnt - kNbrGroup));
-            g[0] = make_uint4(win[0 * BS + threadIdx.x], win[1 * BS + threadIdx.x],
-                              win[2 * BS + threadIdx.x], win[3 * BS + threadIdx.x]);
-            g[1] = make_uint4(win[4 * BS + threadIdx.x], win[5 * BS + threadIdx.x],
-                              win[6 * BS + threadIdx.x], win[7 * BS + threadIdx.x]);
-          }
+          win[(cnt % kRing) * BS + threadIdx.x] = b;
+          cnt += accept;
+        }
+        if (cnt - stored >= kNbrGroup) {
+          if (stored + kNbrGroup <= kMaxNbr) storeGroup(stored, kNbrGroup);
+          stored += kNbrGroup;
         }
       }
     });
@@ -148,15 +175,7 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
     // blocks of kListPad entries: readers load whole blocks and skip the
     // entries past the count, so every loaded index is a valid particle.
     if (cnt <= kMaxNbr) {
-      const uint32_t r = cnt % kNbrGroup;
-      if (r != 0u) {
-        uint32_t e[kNbrGroup];
-#pragma unroll
-        for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = j < r ? win[j * BS + threadIdx.x] : a;
-        uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, cnt - r));
-        g[0] = make_uint4(e[0], e[1], e[2], e[3]);
-        g[1] = make_uint4(e[4], e[5], e[6], e[7]);
-      }
+      if (cnt != stored) storeGroup(stored, cnt - stored);
       for (uint32_t s = (cnt + kNbrGroup - 1) / kNbrGroup * kNbrGroup;
            s % kListPad != 0u; s += kNbrGroup) {
         uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, s));
@@ -208,14 +227,6 @@ struct PosVel {
 };
 __device__ inline const float4& posOf(const float4& r) { return r; }
 __device__ inline const float4& posOf(const PosVel& r) { return r.p; }
-
-// base[i] addressed as base + a 32-bit byte offset (the base stays in scalar
-// registers, no 64-bit address arithmetic per gather). Particle indices are
-// below 2^28 (checked at upload), so i * 16 fits.
-template <typename T> __device__ inline T at(const T* __restrict__ base, uint32_t i) {
-  static_assert(sizeof(T) <= 16, "records of at most 16 bytes");
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(T)));
-}
 
 // Visits the accepted neighbours (r <= h) of particle a at position pa, in
 // the reference's order: through its list when usable, else the 27 cells.
