@@ -83,19 +83,23 @@ __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint3
                                        float y, float z, uint32_t key) {
   const float4 p0 = L.P0[a];
   const float da = __builtin_sqrtf(dist2(x - p0.x, y - p0.y, z - p0.z));
-  uint32_t m = 0;  // float bits of non-negative floats order as integers
   const uint32_t xs[3] = {dilatedDec(key, kMortonX), key & kMortonX, dilatedInc(key, kMortonX)};
   const uint32_t ys[3] = {dilatedDec(key, kMortonY), key & kMortonY, dilatedInc(key, kMortonY)};
   const uint32_t zs[3] = {dilatedDec(key, kMortonZ), key & kMortonZ, dilatedInc(key, kMortonZ)};
+  // all 27 loads issued before any is used (a cell outside the table reads
+  // entry 0 and is masked): one memory latency, not 27 in a row
+  uint32_t d[27];
 #pragma unroll
-  for (int iz = 0; iz < 3; ++iz)
+  for (int i = 0; i < 27; ++i) {
+    const uint32_t code = xs[i % 3] | ys[(i / 3) % 3] | zs[i / 9];
+    d[i] = L.cellDisp[c.keyBase + (code + 1u < c.G ? code : 0u)];
+  }
+  uint32_t m = 0;  // float bits of non-negative floats order as integers
 #pragma unroll
-    for (int iy = 0; iy < 3; ++iy)
-#pragma unroll
-      for (int ix = 0; ix < 3; ++ix) {
-        const uint32_t code = xs[ix] | ys[iy] | zs[iz];
-        if (code + 1u < c.G) m = max(m, L.cellDisp[c.keyBase + code]);
-      }
+  for (int i = 0; i < 27; ++i) {
+    const uint32_t code = xs[i % 3] | ys[(i / 3) % 3] | zs[i / 9];
+    m = max(m, code + 1u < c.G ? d[i] : 0u);
+  }
   return da + __uint_as_float(m) < c.skinSafe;
 }
 
@@ -228,6 +232,29 @@ struct PosVel {
 __device__ inline const float4& posOf(const float4& r) { return r; }
 __device__ inline const float4& posOf(const PosVel& r) { return r.p; }
 
+// The 27-cell walk of a wave whose lists are not usable: candidates loaded
+// kWalkBlock at a time (indices clamped into the cell, so every load is a
+// valid particle) and used in order, with the next cell's table entries
+// loaded during the visit; one memory latency per block, not per candidate.
+constexpr int kWalkBlock = 4;
+template <typename Load, typename Use>
+__device__ inline void walkCells(const PbfConsts& c, float4 pa, uint32_t key,
+                                 const uint32_t* __restrict__ table, Load&& load, Use&& use) {
+  forEachNeighbourCellRolled(key, table, c.G, [&](uint32_t s, uint32_t e) {
+    for (uint32_t b0 = s; b0 < e; b0 += kWalkBlock) {
+      decltype(load(0u)) p[kWalkBlock];
+#pragma unroll
+      for (int k = 0; k < kWalkBlock; ++k) p[k] = load(min(b0 + k, e - 1u));
+#pragma unroll
+      for (int k = 0; k < kWalkBlock; ++k) {
+        const float4& q = posOf(p[k]);
+        const float d2 = dist2(pa.x - q.x, pa.y - q.y, pa.z - q.z);
+        if (b0 + k < e && d2 <= c.rmax2) use(b0 + k, p[k], d2);
+      }
+    }
+  });
+}
+
 // Visits the accepted neighbours (r <= h) of particle a at position pa, in
 // the reference's order: through its list when usable, else the 27 cells.
 // load(b) returns the candidate's float4 (xyz = p*). The list is walked in
@@ -274,14 +301,63 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
       }
     }
   } else {
-    forEachNeighbourCell(key, table, c.G, [&](uint32_t s, uint32_t e) {
-      for (uint32_t b = s; b < e; ++b) {
-        const auto pb = load(b);
-        const float4& q = posOf(pb);
-        const float d2 = dist2(pa.x - q.x, pa.y - q.y, pa.z - q.z);
-        if (d2 <= c.rmax2) use(b, pb, d2);
+    walkCells(c, pa, key, table, load, use);
+  }
+}
+
+#ifndef LF_XSPH_SPLIT
+#define LF_XSPH_SPLIT 1
+#endif
+// forEachAccepted for a pass that needs a second record (V) of its accepted
+// neighbours only: each block gathers the positions, then V of the entries
+// within h (lanes whose entry lies in the skin issue no request), then runs
+// use(p_b, v_b, d2) in list order. Same neighbours, same order.
+template <int BLK, typename Use>
+__device__ inline void forEachAcceptedSplit(const PbfConsts& c, const Lists& L, int pass,
+                                            uint32_t a, float4 pa, uint32_t key,
+                                            const uint32_t* __restrict__ table,
+                                            const float4* __restrict__ P,
+                                            const float4* __restrict__ V, Use&& use) {
+  uint32_t cnt;
+  bool usable = listUsable(c, L, a, pa.x, pa.y, pa.z, false, cnt);
+  if (!usable && cnt <= kMaxNbr) usable = listUsableLocal(c, L, a, pa.x, pa.y, pa.z, key);
+  const uint64_t fell = __ballot(!usable);
+  if (fell && __lane_id() == __builtin_ctzll(fell)) atomicAdd(&L.maxDisp[pass], uint32_t(__popcll(fell)));
+  if (fell) usable = false;  // the whole wave walks cells (as forEachAccepted)
+  if (usable) {
+    static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
+    static_assert(kListPad % BLK == 0, "the build pads lists to whole blocks");
+    for (uint32_t i = 0; i < cnt; i += BLK) {
+      uint32_t b[BLK];
+#pragma unroll
+      for (int k = 0; k < BLK; k += 4) {  // 16-byte loads; entries past cnt hold a
+        const uint4 q = *reinterpret_cast<const uint4*>(L.nbr + nbrSlot(a, i + k));
+        b[k] = q.x;
+        b[k + 1] = q.y;
+        b[k + 2] = q.z;
+        b[k + 3] = q.w;
       }
-    });
+      float4 p[BLK];
+#pragma unroll
+      for (int k = 0; k < BLK; ++k) p[k] = at(P, b[k]);
+      float d2[BLK];
+      bool in[BLK];
+      float4 v[BLK];
+#pragma unroll
+      for (int k = 0; k < BLK; ++k) {
+        d2[k] = dist2(pa.x - p[k].x, pa.y - p[k].y, pa.z - p[k].z);
+        in[k] = i + k < cnt && d2[k] <= c.rmax2;
+        v[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (in[k]) v[k] = at(V, b[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < BLK; ++k)
+        if (in[k]) use(p[k], v[k], d2[k]);
+    }
+  } else {
+    walkCells(
+        c, pa, key, table, [&](uint32_t b) { return PosVel{at(P, b), at(V, b)}; },
+        [&](uint32_t, const PosVel& r, float d2) { use(r.p, r.v, d2); });
   }
 }
 
@@ -550,20 +626,27 @@ __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict_
     }
     const float4 pa = P[a];
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    auto term = [&](const float4& pb, const float4& vb, float d2) {
+      // P.w is the fluid mass, 0 for obstacles: they add zero-weight terms.
+      // W6 on d2 (no square root).
+      const float t = c.hh - d2;
+      const float mw = pb.w * (c.poly6 * ((t * t) * t));
+      ax = __builtin_fmaf(mw, vb.x - va.x, ax);
+      ay = __builtin_fmaf(mw, vb.y - va.y, ay);
+      az = __builtin_fmaf(mw, vb.z - va.z, az);
+    };
+#if LF_XSPH_SPLIT
+    // List walk in two gathers per block: positions, then the velocities of
+    // the entries within h only (about half the list lies in the skin).
+    forEachAcceptedSplit<kBlockXsph>(
+        c, L, 3, a, pa, keys[a], table + c.keyBase, P, Vin,
+        [&](const float4& pb, const float4& vb, float d2) { term(pb, vb, d2); });
+#else
     forEachAccepted<kBlockXsph>(
         c, L, 3, false, a, pa, keys[a], table + c.keyBase,
         [&](uint32_t b) { return PosVel{at(P, b), at(Vin, b)}; },
-        [&](uint32_t, const PosVel& r, float d2) {
-          const float4& pb = r.p;
-          // P.w is the fluid mass, 0 for obstacles: they add zero-weight terms.
-          // W6 on d2 (no square root).
-          const float t = c.hh - d2;
-          const float mw = pb.w * (c.poly6 * ((t * t) * t));
-          const float4& vb = r.v;
-          ax = __builtin_fmaf(mw, vb.x - va.x, ax);
-          ay = __builtin_fmaf(mw, vb.y - va.y, ay);
-          az = __builtin_fmaf(mw, vb.z - va.z, az);
-        });
+        [&](uint32_t, const PosVel& r, float d2) { term(r.p, r.v, d2); });
+#endif
     Vout[a] = make_float4(__builtin_fmaf(c.xsphRho, ax, va.x),
                           __builtin_fmaf(c.xsphRho, ay, va.y),
                           __builtin_fmaf(c.xsphRho, az, va.z), va.w);
