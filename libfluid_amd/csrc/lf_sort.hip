@@ -7,8 +7,10 @@
 // Keys are 30-bit Morton codes (10 bits per axis, curves.h:56-72): three
 // passes of 10-bit digits. Each pass is three kernels over tiles of 2048
 // pairs (256 threads x 8):
-//   hist    per-tile digit counts (wave-aggregated LDS counters), digit-major
-//   scan    per-digit exclusive scan over tiles (one workgroup per digit)
+//   hist    per-tile digit counts (wave-aggregated LDS counters), stored
+//           tile-major: one contiguous 4 KB row per tile
+//   scan    per-digit exclusive scan over tiles (8 digits x 128 tile
+//           segments per workgroup)
 //   scatter digit bases + stable in-tile rank (per-wave ballot match, LDS
 //           running counters in (wave, slot, lane) = input order) -> output
 #include "lf_internal.hpp"
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t* __restr
     if (valid && __lane_id() == leader) atomicAdd(&h[d], uint32_t(__popcll(peers)));
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kRadix; i += kThreads) hist[i * nTiles + blockIdx.x] = h[i];
+  for (uint32_t i = threadIdx.x; i < kRadix; i += kThreads) hist[size_t(blockIdx.x) * kRadix + i] = h[i];
 }
 
 // Block-wide exclusive scan of one value per thread (256 threads); returns
@@ -88,32 +90,62 @@ __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t* sm /*[kWaves
   return waveBase + x - v;
 }
 
-// One workgroup per digit: exclusive scan of hist[d][0..nTiles) in place,
-// digit total into tot[d].
-__global__ __launch_bounds__(kThreads) void k_radix_scan(uint32_t* __restrict__ hist,
-                                                         uint32_t nTiles,
-                                                         uint32_t* __restrict__ tot) {
-  __shared__ uint32_t sm[kWaves];
-  uint32_t* row = hist + size_t(blockIdx.x) * nTiles;
-  const uint32_t per = (nTiles + kThreads - 1) / kThreads;
-  const uint32_t lo = threadIdx.x * per;
-  const uint32_t hi = min(lo + per, nTiles);
+// Exclusive scan over tiles of each digit's counts, in place (hist[t][d]),
+// digit totals into tot[d]. A workgroup of kScanThreads takes kScanDigits
+// digits; its threads split the tiles into kScanSegs segments per digit (a few
+// tiles each), so each load of a wave reads 32-byte runs of 8 digits.
+constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanDigits = 8;
+constexpr uint32_t kScanSegs = kScanThreads / kScanDigits;
+static_assert(kRadix % kScanDigits == 0, "whole digit groups per workgroup");
+__global__ __launch_bounds__(kScanThreads) void k_radix_scan(uint32_t* __restrict__ hist,
+                                                             uint32_t nTiles,
+                                                             uint32_t* __restrict__ tot) {
+  static_assert(kScanSegs == 128, "a digit's segment sums are scanned by two waves");
+  __shared__ uint32_t segSum[kScanSegs][kScanDigits];
+  __shared__ uint32_t waveTot[kScanThreads / 64];
+  const uint32_t dl = threadIdx.x % kScanDigits, seg = threadIdx.x / kScanDigits;
+  const uint32_t d = blockIdx.x * kScanDigits + dl;
+  const uint32_t per = (nTiles + kScanSegs - 1) / kScanSegs;
+  const uint32_t t0 = min(seg * per, nTiles), t1 = min(t0 + per, nTiles);
   uint32_t s = 0;
-  for (uint32_t i = lo; i < hi; ++i) s += row[i];
-  uint32_t total;
-  uint32_t run = blockExclusiveScan(s, sm, &total);
-  for (uint32_t i = lo; i < hi; ++i) {
-    const uint32_t t = row[i];
-    row[i] = run;
-    run += t;
+#pragma unroll 4
+  for (uint32_t t = t0; t < t1; ++t) s += hist[size_t(t) * kRadix + d];
+  segSum[seg][dl] = s;
+  __syncthreads();
+  {
+    // exclusive prefix of the segment sums: thread j scans segment j % 128 of
+    // digit j / 128 (two waves per digit, a shuffle scan in each)
+    const uint32_t sd = threadIdx.x / kScanSegs, ss = threadIdx.x % kScanSegs;
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    const uint32_t v = segSum[ss][sd];
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63u) waveTot[w] = x;
+    __syncthreads();
+    const uint32_t before = (ss >= 64u) ? waveTot[w - 1] : 0u;
+    segSum[ss][sd] = before + x - v;
+    if (ss == kScanSegs - 1) tot[blockIdx.x * kScanDigits + sd] = before + x;
   }
-  if (threadIdx.x == 0) tot[blockIdx.x] = total;
+  __syncthreads();
+  uint32_t run = segSum[seg][dl];
+#pragma unroll 4
+  for (uint32_t t = t0; t < t1; ++t) {
+    const size_t i = size_t(t) * kRadix + d;
+    const uint32_t v = hist[i];
+    hist[i] = run;
+    run += v;
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int shift,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot, uint32_t nTiles) {
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot) {
   __shared__ uint32_t cnt[kWaves][kRadix];  // per-wave counts -> wave offsets
   __shared__ uint32_t base[kRadix];         // this tile's global base per digit
   __shared__ uint32_t sm[kWaves];
@@ -134,7 +166,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const uint32_t d = threadIdx.x * kPer + i;
-    base[d] = run + hist[size_t(d) * nTiles + blockIdx.x];
+    base[d] = run + hist[size_t(blockIdx.x) * kRadix + d];
     run += t4[i];
   }
 
@@ -210,13 +242,13 @@ int radixSort(Ctx* c, size_t n, int& outIdx) {
     rc = launch(c, "radix_hist", k_radix_hist, dim3(nTiles), dim3(kThreads), 0,
                 (const uint32_t*)c->sortKey[src].p, uint32_t(n), shift, hist, nTiles);
     if (rc) return rc;
-    rc = launch(c, "radix_scan", k_radix_scan, dim3(kRadix), dim3(kThreads), 0, hist, nTiles,
-                tot);
+    rc = launch(c, "radix_scan", k_radix_scan, dim3(kRadix / kScanDigits), dim3(kScanThreads), 0,
+                hist, nTiles, tot);
     if (rc) return rc;
     rc = launch(c, "radix_scatter", k_radix_scatter, dim3(nTiles), dim3(kThreads), 0,
                 (const uint32_t*)c->sortKey[src].p, (const uint32_t*)c->sortVal[src].p,
                 c->sortKey[1 - src].p, c->sortVal[1 - src].p, uint32_t(n), shift,
-                (const uint32_t*)hist, (const uint32_t*)tot, nTiles);
+                (const uint32_t*)hist, (const uint32_t*)tot);
     if (rc) return rc;
     src = 1 - src;
   }
