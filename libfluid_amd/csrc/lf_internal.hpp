@@ -130,17 +130,46 @@ int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string
 // cells instead; results are the same either way.
 constexpr uint32_t kMaxNbr = 128;
 
-// Slot of entry i of particle a's list: groups of 8 entries (32 bytes) per
-// particle, the 64 particles of a wave side by side within a group, groups in
-// entry order. A wave reading entries i..i+7 of its 64 particles reads one
-// contiguous 2 KB run (two 16-byte loads per lane); the build stores a full
-// 32-byte group per lane at a time.
+// List storage. Entry i of particle a holds its neighbour b as the 24-bit
+// value b - a + 2^23 (neighbours more than 2^23 sorted positions away mark the
+// list overflowed, so a context of up to 2^23 + 1 particles never needs it).
+// Entries come in groups of 8 (24 bytes, two 12-byte loads per lane), the 64
+// particles of a wave side by side within a group (1,536 contiguous bytes),
+// groups in entry order. Entries are 3/4 the size of 32-bit indices: the pair
+// passes stream the lists, most of their memory traffic.
 constexpr uint32_t kNbrGroup = 8;
-__host__ __device__ constexpr uint64_t nbrSlot(uint32_t a, uint32_t i) {
-  return ((uint64_t(a >> 6) * (kMaxNbr / kNbrGroup) + (i / kNbrGroup)) << 9) |
-         (uint64_t(a & 63u) << 3) | (i % kNbrGroup);
+constexpr uint32_t kNbrGroupBytes = kNbrGroup * 3;           // 24
+constexpr uint32_t kNbrWaveGroupBytes = 64 * kNbrGroupBytes;  // 1536
+constexpr uint32_t kNbrBias = 1u << 23;
+// Byte offset of group g (entries 8g .. 8g+7) of particle a's list.
+__host__ __device__ constexpr uint64_t nbrGroupByte(uint32_t a, uint32_t g) {
+  return (uint64_t(a >> 6) * (kMaxNbr / kNbrGroup) + g) * kNbrWaveGroupBytes +
+         uint64_t(a & 63u) * kNbrGroupBytes;
 }
-inline size_t nbrCapacity(size_t n) { return ((n + 63) / 64) * 64 * kMaxNbr; }
+// Capacity in 32-bit words.
+inline size_t nbrCapacity(size_t n) {
+  return ((n + 63) / 64) * (kMaxNbr / kNbrGroup) * kNbrWaveGroupBytes / 4;
+}
+// Packing of 8 biased entries into 6 words and back.
+__device__ inline void packGroup(const uint32_t e[kNbrGroup], uint32_t w[6]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t* x = e + 4 * h;
+    w[3 * h + 0] = x[0] | (x[1] << 24);
+    w[3 * h + 1] = (x[1] >> 8) | (x[2] << 16);
+    w[3 * h + 2] = (x[2] >> 16) | (x[3] << 8);
+  }
+}
+__device__ inline void unpackGroup(const uint32_t w[6], uint32_t e[kNbrGroup]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t* x = w + 3 * h;
+    e[4 * h + 0] = x[0] & 0xFFFFFFu;
+    e[4 * h + 1] = ((x[0] >> 24) | (x[1] << 8)) & 0xFFFFFFu;
+    e[4 * h + 2] = ((x[1] >> 16) | (x[2] << 16)) & 0xFFFFFFu;
+    e[4 * h + 3] = x[2] >> 8;
+  }
+}
 
 // ---- Morton codes (curves.h:56-72), 10 bits per axis ----------------------
 __host__ __device__ inline uint32_t spread10(uint32_t v) {
@@ -429,7 +458,7 @@ struct Ctx {
   DevBuf<uint32_t> sortKey[2], sortVal[2];
   DevBuf<float4> P, PL;       // p* (+mass), p* + lambda
   DevBuf<float> rho;          // density of the last lambda pass
-  DevBuf<uint32_t> nbr;       // neighbour lists, nbrSlot(a, i)
+  DevBuf<uint32_t> nbr;       // neighbour lists (nbrGroupByte layout)
   DevBuf<uint32_t> nbrCount;  // list length per particle (> kMaxNbr: overflowed)
   DevBuf<float4> P0;          // p* at the list build
   DevBuf<uint32_t> maxDisp;   // [0]: float bits of the largest displacement since

@@ -56,7 +56,7 @@ static_assert(kMaxNbr % kListPad == 0, "padding stays within the list capacity")
 
 
 struct Lists {
-  uint32_t* nbr;           // nbrSlot(a, i), kMaxNbr entries per particle
+  uint32_t* nbr;           // 24-bit entries, nbrGroupByte layout, kMaxNbr per particle
   uint32_t* count;         // entries per particle (> kMaxNbr: overflowed)
   float4* P0;              // clamped p* at the build
   uint32_t* maxDisp;       // [0]: float bits of D, the largest |p* - P0| so far;
@@ -122,7 +122,8 @@ template <typename T> __device__ inline T at(const T* __restrict__ base, uint32_
 template <int BS>
 __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ table,
-                                                   const uint8_t* __restrict__ T, Lists L) {
+                                                   const uint8_t* __restrict__ T, Lists L,
+                                                   int checkFar) {
   constexpr uint32_t kRing = 2 * kNbrGroup;
   static_assert(kBuildBlock < int(kNbrGroup), "a block completes at most one group");
   // win[j * BS + lane] holds entry j mod 16 of the lane's list; lanes side by
@@ -138,20 +139,24 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
     }
     const float4 pa = L.P0[a];
     uint32_t cnt = 0, stored = 0;  // entries accepted / written to the list (a multiple of 8)
-    // group i / 8 of a's list starts at nbrSlot(a, 0) + (i / 8) * 512 entries
-    char* const list0 = reinterpret_cast<char*>(L.nbr + nbrSlot(a, 0));
-    static_assert(nbrSlot(0, kNbrGroup) == 512u, "groups of a particle 512 entries apart");
+    bool far = false;  // a candidate 2^23 or more sorted positions away: no list
+    // group i / 8 of a's list starts at nbrGroupByte(a, 0) + (i / 8) * 1536 bytes
+    char* const list0 = reinterpret_cast<char*>(L.nbr) + nbrGroupByte(a, 0);
+    static_assert(nbrGroupByte(0, 1) == kNbrWaveGroupBytes, "groups of a particle 1536 bytes apart");
     auto storeGroup = [&](uint32_t first, uint32_t valid) {  // entries [first, first + 8)
       // first is a multiple of 8: its slots are (first & 8) + 0..7, no wrap
       const uint32_t* w = win + (first & kNbrGroup) * BS + threadIdx.x;
-      uint32_t e[kNbrGroup];
+      uint32_t e[kNbrGroup], q[6];
 #pragma unroll
-      for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = j < valid ? w[j * BS] : a;
-      uint4* g = reinterpret_cast<uint4*>(list0 + (first / kNbrGroup) * (512u * 4u));
-      g[0] = make_uint4(e[0], e[1], e[2], e[3]);
-      g[1] = make_uint4(e[4], e[5], e[6], e[7]);
+      for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = (j < valid ? w[j * BS] : a) - a + kNbrBias;
+      packGroup(e, q);
+      uint3* g = reinterpret_cast<uint3*>(list0 + (first / kNbrGroup) * kNbrWaveGroupBytes);
+      g[0] = make_uint3(q[0], q[1], q[2]);
+      g[1] = make_uint3(q[3], q[4], q[5]);
     };
     forEachNeighbourCellRolled(keys[a], table + c.keyBase, c.G, [&](uint32_t s, uint32_t e) {
+      // a cell's candidates are the run [s, e): its ends bound their distance
+      if (checkFar && ((s - a + kNbrBias) >> 24 || (e - 1u - a + kNbrBias) >> 24)) far = true;
       for (uint32_t b0 = s; b0 < e; b0 += kBuildBlock) {
         float4 pb[kBuildBlock];  // candidates loaded together (indices clamped into the cell)
 #pragma unroll
@@ -178,16 +183,13 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
     // The last cnt % 8 entries, and the list padded with a itself to whole
     // blocks of kListPad entries: readers load whole blocks and skip the
     // entries past the count, so every loaded index is a valid particle.
-    if (cnt <= kMaxNbr) {
+    if (cnt <= kMaxNbr && !far) {
       if (cnt != stored) storeGroup(stored, cnt - stored);
       for (uint32_t s = (cnt + kNbrGroup - 1) / kNbrGroup * kNbrGroup;
-           s % kListPad != 0u; s += kNbrGroup) {
-        uint4* g = reinterpret_cast<uint4*>(L.nbr + nbrSlot(a, s));
-        g[0] = make_uint4(a, a, a, a);
-        g[1] = make_uint4(a, a, a, a);
-      }
+           s % kListPad != 0u; s += kNbrGroup)
+        storeGroup(s, 0);
     }
-    L.count[a] = cnt;
+    L.count[a] = far ? kMaxNbr + 1u : cnt;
   });
 }
 
@@ -231,6 +233,26 @@ struct PosVel {
 };
 __device__ inline const float4& posOf(const float4& r) { return r; }
 __device__ inline const float4& posOf(const PosVel& r) { return r.p; }
+
+// Entries i .. i+BLK-1 of particle a's list as particle indices: two 12-byte
+// loads per group of 8, unpacked, b = a + entry - 2^23.
+template <int BLK>
+__device__ inline void readListBlock(const Lists& L, uint32_t a, uint32_t i, uint32_t b[BLK]) {
+  static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
+  const char* const list0 = reinterpret_cast<const char*>(L.nbr) + nbrGroupByte(a, 0);
+  const uint32_t base = a - kNbrBias;
+#pragma unroll
+  for (int g = 0; g < BLK / int(kNbrGroup); ++g) {
+    const uint3* q =
+        reinterpret_cast<const uint3*>(list0 + (i / kNbrGroup + g) * kNbrWaveGroupBytes);
+    const uint3 lo = q[0], hi = q[1];
+    const uint32_t w[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+    uint32_t e[kNbrGroup];
+    unpackGroup(w, e);
+#pragma unroll
+    for (int j = 0; j < int(kNbrGroup); ++j) b[g * kNbrGroup + j] = base + e[j];
+  }
+}
 
 // The 27-cell walk of a wave whose lists are not usable: candidates loaded
 // kWalkBlock at a time (indices clamped into the cell, so every load is a
@@ -282,14 +304,7 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
     static_assert(kListPad % BLK == 0, "the build pads lists to whole blocks");
     for (uint32_t i = 0; i < cnt; i += BLK) {
       uint32_t b[BLK];
-#pragma unroll
-      for (int k = 0; k < BLK; k += 4) {  // 16-byte loads; entries past cnt hold a
-        const uint4 q = *reinterpret_cast<const uint4*>(L.nbr + nbrSlot(a, i + k));
-        b[k] = q.x;
-        b[k + 1] = q.y;
-        b[k + 2] = q.z;
-        b[k + 3] = q.w;
-      }
+      readListBlock<BLK>(L, a, i, b);  // entries past cnt hold a
       decltype(load(0u)) p[BLK];
 #pragma unroll
       for (int k = 0; k < BLK; ++k) p[k] = load(b[k]);
@@ -329,14 +344,7 @@ __device__ inline void forEachAcceptedSplit(const PbfConsts& c, const Lists& L, 
     static_assert(kListPad % BLK == 0, "the build pads lists to whole blocks");
     for (uint32_t i = 0; i < cnt; i += BLK) {
       uint32_t b[BLK];
-#pragma unroll
-      for (int k = 0; k < BLK; k += 4) {  // 16-byte loads; entries past cnt hold a
-        const uint4 q = *reinterpret_cast<const uint4*>(L.nbr + nbrSlot(a, i + k));
-        b[k] = q.x;
-        b[k + 1] = q.y;
-        b[k + 2] = q.z;
-        b[k + 3] = q.w;
-      }
+      readListBlock<BLK>(L, a, i, b);  // entries past cnt hold a
       float4 p[BLK];
 #pragma unroll
       for (int k = 0; k < BLK; ++k) p[k] = at(P, b[k]);
@@ -665,7 +673,8 @@ int pbfBuildLists(Ctx* c) {
   const Work W = workOf(c, kSlotList);
   return LF_DISPATCH_BS(c, "pbf_build_list", k_build_list, W,
                         (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
-                        (const uint8_t*)c->T[c->cur].p, listsOf(c));
+                        (const uint8_t*)c->T[c->cur].p, listsOf(c),
+                        int(c->n > size_t(kNbrBias) + 1));
 }
 
 // Calls f with std::integral_constant arguments for the workgroup size and

@@ -13,7 +13,7 @@ import pytest
 import libfluid_amd as lf
 import oracle
 from libfluid_amd import scenes
-from libfluid_amd._abi import (LF_ARR_CASES, LF_ARR_DENSITY, LF_ARR_DIFFUSED, LF_ARR_FIELD,
+from libfluid_amd._abi import (LF_ARR_LISTSTATS, LF_ARR_CASES, LF_ARR_DENSITY, LF_ARR_DIFFUSED, LF_ARR_FIELD,
                                LF_ARR_KEYS, LF_ARR_LAMBDA, LF_ARR_MCCOLOUR, LF_ARR_MCNORMAL,
                                LF_ARR_ORDER, LF_ARR_POSITION, LF_ARR_PSTAR, LF_ARR_TABLE,
                                LF_ARR_VELOCITY, LF_ERR_MORTON)
@@ -152,6 +152,28 @@ def test_step_1m_bit_exact(ctx):
     ofield = o.mc_density(m.samples)
     assert (field > 0).sum() > 1_000_000
     _eq(field.astype(np.float64).reshape(ofield.shape), ofield, "density field 1M 256^3")
+
+
+def test_far_neighbours_fall_back_bit_exact(ctx):
+    """List entries are 24-bit offsets from the particle's own sorted position;
+    a particle with a candidate 2^23 or more positions away (possible only in a
+    context of more than 2^23 + 1 particles) walks its cells instead. A 272^3
+    block, 20,123,648 particles, where neighbours across the coarsest Morton
+    planes lie ~10M positions apart: one step (one iteration) matches the
+    oracle bit for bit, and some lists did fall back."""
+    s = scenes.SOLVER_SPACING
+    sc = scenes._scene("block_20m", [((272, 272, 272), (s / 2,) * 3)], s, (27.2, 14.0, 14.0),
+                       11, 1, (8, 8, 8))
+    p = sc.params(iterations=1)
+    o = _pair(ctx, sc, p)
+    ctx.step(p, 1)
+    stats = ctx.read(LF_ARR_LISTSTATS)
+    o.step(1)
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position 20M")
+    _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity 20M")
+    assert stats[1] > 0, "no lambda-pass lane walked cells"
+    del o
 
 
 def test_density_mc_bit_exact(ctx):
