@@ -143,12 +143,13 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
     // group i / 8 of a's list starts at nbrGroupByte(a, 0) + (i / 8) * 1536 bytes
     char* const list0 = reinterpret_cast<char*>(L.nbr) + nbrGroupByte(a, 0);
     static_assert(nbrGroupByte(0, 1) == kNbrWaveGroupBytes, "groups of a particle 1536 bytes apart");
+    const uint32_t bias = kNbrBias - a;  // ring entries are stored biased: b - a + 2^23
     auto storeGroup = [&](uint32_t first, uint32_t valid) {  // entries [first, first + 8)
       // first is a multiple of 8: its slots are (first & 8) + 0..7, no wrap
       const uint32_t* w = win + (first & kNbrGroup) * BS + threadIdx.x;
       uint32_t e[kNbrGroup], q[6];
 #pragma unroll
-      for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = (j < valid ? w[j * BS] : a) - a + kNbrBias;
+      for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = j < valid ? w[j * BS] : kNbrBias;
       packGroup(e, q);
       uint3* g = reinterpret_cast<uint3*>(list0 + (first / kNbrGroup) * kNbrWaveGroupBytes);
       g[0] = make_uint3(q[0], q[1], q[2]);
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
               (dist2(pa.x - pb[k].x, pa.y - pb[k].y, pa.z - pb[k].z) <= c.listR2 ? 1u : 0u);
           // written whether accepted or not: slot cnt % 16 is the next free
           // one, so a rejected candidate is overwritten by the next accepted
-          win[(cnt % kRing) * BS + threadIdx.x] = b;
+          win[(cnt % kRing) * BS + threadIdx.x] = b + bias;
           cnt += accept;
         }
         if (cnt - stored >= kNbrGroup) {
