@@ -102,17 +102,17 @@ void timingBegin(Ctx* c, const char* name, PendingTiming& t) {
     }
     (i == 0 ? t.start : t.stop) = e;
   }
-  if (t.start) (void)hipEventRecord(t.start, c->stream);
+  if (t.start) (void)hipEventRecord(t.start, c->launchStream ? c->launchStream : c->stream);
 }
 
 void timingEnd(Ctx* c, PendingTiming& t) {
-  if (t.stop) (void)hipEventRecord(t.stop, c->stream);
+  if (t.stop) (void)hipEventRecord(t.stop, c->launchStream ? c->launchStream : c->stream);
   if (t.start && t.stop) c->pending.push_back(t);
 }
 
 static void drainTimings(Ctx* c) {
   if (c->pending.empty()) return;
-  (void)hipStreamSynchronize(c->stream);
+  syncStreams(c);
   for (PendingTiming& t : c->pending) {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
@@ -142,6 +142,7 @@ static int setParams(Ctx* c, const lf_pbf_params* p) {
 // keyBase_s + G_s] with out-of-table keys clamped to G_s. Uploads happen only
 // when something changed, never during graph capture.
 int prepareScenes(Ctx* c) {
+  c->scenesUploaded = false;
   std::vector<PbfConsts> consts;
   std::vector<uint32_t> begin;
   uint64_t total = 0;
@@ -181,6 +182,7 @@ int prepareScenes(Ctx* c) {
     return fail(c, LF_ERR_STATE, "parameters or sizes changed during graph capture");
   int rc;
   dropGraph(c);
+  c->scenesUploaded = true;
   if (!sameConsts) {
     if ((rc = ensure(c, c->dConsts, consts.size(), "scene constants"))) return rc;
     LF_HIP(c, hipMemcpyAsync(c->dConsts.p, consts.data(), consts.size() * sizeof(PbfConsts),
@@ -298,8 +300,13 @@ static int appendSources(Ctx* c) {
 
 static int pbfStepOnce(Ctx* c) {
   int rc;
+  // the key and sort kernels may overlap the main stream's work since the
+  // last step (its marching cubes) when nothing since then wrote their inputs
+  const bool overlap = c->stepInputsKept && !c->capturing && c->sources.empty() &&
+                       c->hostDrains.empty() && c->nQueries == 0;
   if ((rc = appendSources(c))) return rc;
   if ((rc = applyDrains(c))) return rc;
+  c->overlapSort = overlap;  // consumed by gridBuild
   if ((rc = gridBuild(c))) return rc;
   if (c->nQueries && (rc = answerQueries(c))) return rc;
   if ((rc = pbfBuildLists(c))) return rc;
@@ -317,6 +324,10 @@ static int pbfStepOnce(Ctx* c) {
   if (!fuse && (rc = pbfFinalise(c))) return rc;
   if ((rc = pbfXsph(c))) return rc;
   c->stepDone = true;
+  if (!c->capturing) {
+    LF_HIP(c, hipEventRecord(c->evStepEnd, c->stream));
+    c->stepInputsKept = true;
+  }
   return LF_OK;
 }
 
@@ -363,6 +374,16 @@ int lf_ctx_create(int device, lf_ctx** out) {
     delete c;
     return LF_ERR_HIP;
   }
+  // default priority, like the main stream; timing-free events for ordering
+  if (hipStreamCreateWithFlags(&c->sortStream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evStepEnd, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evSorted, hipEventDisableTiming) != hipSuccess) {
+    if (c->evStepEnd) (void)hipEventDestroy(c->evStepEnd);
+    if (c->sortStream) (void)hipStreamDestroy(c->sortStream);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LF_ERR_HIP;
+  }
   *out = c;
   return LF_OK;
 }
@@ -370,7 +391,7 @@ int lf_ctx_create(int device, lf_ctx** out) {
 int lf_ctx_destroy(lf_ctx* c) {
   if (!c) return LF_OK;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
+  lf::syncStreams(c);
   for (auto& t : c->pending) {
     (void)hipEventDestroy(t.start);
     (void)hipEventDestroy(t.stop);
@@ -425,6 +446,9 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->sceneDone);
   lf::release(c->scenePasses);
   lf::dropGraph(c);
+  (void)hipEventDestroy(c->evStepEnd);
+  (void)hipEventDestroy(c->evSorted);
+  (void)hipStreamDestroy(c->sortStream);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return LF_OK;
@@ -435,12 +459,14 @@ const char* lf_last_error(const lf_ctx* c) { return c ? c->err.c_str() : "null c
 int lf_sync(lf_ctx* c) {
   if (!c) return LF_ERR_ARG;
   LF_HIP(c, hipStreamSynchronize(c->stream));
+  LF_HIP(c, hipStreamSynchronize(c->sortStream));
   return LF_OK;
 }
 
 int lf_state_upload(lf_ctx* c, size_t n, const float* pos, const float* vel, const float* mass,
                     const int32_t* type, const uint32_t* colour) {
   if (!c) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (n > 0 && (!pos || !vel || !mass || !type || !colour))
     return fail(c, LF_ERR_ARG, "lf_state_upload: null array");
   if (n >= (size_t(1) << 28))  // gathers address particles with 32-bit byte offsets
@@ -529,6 +555,7 @@ int lf_state_size(const lf_ctx* c, size_t* n) {
 
 int lf_set_wells(lf_ctx* c, size_t n, const lf_well* wells) {
   if (!c) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (n > 0 && !wells) return fail(c, LF_ERR_ARG, "lf_set_wells: null wells");
   LF_HIP(c, hipSetDevice(c->device));
   if (n > 0) {
@@ -771,6 +798,7 @@ int lf_advance(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m, lf_part
                size_t n, lf_mesh_triangle* tris, size_t tri_cap, size_t* ntri,
                uint32_t* orderOut) {
   if (!c || !p) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (ntri) *ntri = 0;
   if (n == 0) return LF_OK;  // oclsph.hpp:711-715: empty input, empty result
   if (!xs) return fail(c, LF_ERR_ARG, "lf_advance: null particles");
@@ -864,6 +892,7 @@ int lf_read(lf_ctx* c, int array, void* dst, size_t bytes) {
 
 int lf_write(lf_ctx* c, int array, const void* src, size_t bytes) {
   if (!c || (!src && bytes)) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   LF_HIP(c, hipSetDevice(c->device));
   void* dst = nullptr;
   size_t avail = 0;
@@ -957,6 +986,7 @@ int lf_batch_upload(lf_ctx* c, uint32_t nScenes, const lf_pbf_params* params,
                     const size_t* counts, const float* pos, const float* vel, const float* mass,
                     const int32_t* type, const uint32_t* colour) {
   if (!c) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (nScenes == 0 || !params || !counts) return fail(c, LF_ERR_ARG, "lf_batch_upload: no scenes");
   size_t n = 0;
   std::vector<uint32_t> begin(nScenes + 1, 0u);
@@ -1022,6 +1052,7 @@ int lf_batch_select(lf_ctx* c, uint32_t scene) {
 
 int lf_exec_config(lf_ctx* c, uint32_t threads, int persistent, float skin) {
   if (!c) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (threads != 128 && threads != 256 && threads != 512)
     return fail(c, LF_ERR_ARG, "lf_exec_config: threads must be 128, 256 or 512");
   if (!(skin >= 0.0f && skin <= 1.0f))
@@ -1037,6 +1068,7 @@ int lf_exec_config(lf_ctx* c, uint32_t threads, int persistent, float skin) {
 
 int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
   if (!c || steps == 0) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (c->n == 0 || !(c->params.h > 0.0f))
     return fail(c, LF_ERR_STATE, "lf_autotune: needs a state and one step (for its parameters)");
   LF_HIP(c, hipSetDevice(c->device));
@@ -1132,6 +1164,7 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
 
 int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
   if (!c) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (nsteps == 0 || (nsteps & 1u))
     return fail(c, LF_ERR_ARG, "lf_graph_capture: nsteps must be even and > 0");
   if (!c->stepDone || c->n == 0)
@@ -1177,6 +1210,7 @@ int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
 
 int lf_graph_launch(lf_ctx* c, uint32_t times) {
   if (!c) return LF_ERR_ARG;
+  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (!c->graph) return fail(c, LF_ERR_STATE, "lf_graph_launch: no captured graph");
   LF_HIP(c, hipSetDevice(c->device));
   for (uint32_t t = 0; t < times; ++t) LF_HIP(c, hipGraphLaunch(c->graph, c->stream));

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -448,6 +449,19 @@ struct PendingTiming {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // The next step's key and sort kernels may run on sortStream while the main
+  // stream still runs the marching cubes of the step before (they touch
+  // disjoint buffers). stepInputsKept: since the end of the last step
+  // (evStepEnd), the main stream has received nothing that writes the
+  // buffers those kernels read (state, wells, constants, chunks) or the sort
+  // buffers; kernels other than marching cubes, wire packing and diagnostics
+  // clear it, and so do the entry points that write state or wells.
+  hipStream_t sortStream = nullptr;
+  hipEvent_t evStepEnd = nullptr, evSorted = nullptr;
+  hipStream_t launchStream = nullptr;  // non-null: launch() targets this stream
+  bool stepInputsKept = false;
+  bool scenesUploaded = false;  // the last prepareScenes copied constants or chunks
+  bool overlapSort = false;     // this step's key and sort kernels run on sortStream
   std::string err;
 
   size_t n = 0;
@@ -566,10 +580,18 @@ int hipFail(Ctx* c, hipError_t e, const char* where);
     if (_e != hipSuccess) return ::lf::hipFail((ctx), _e, #expr); \
   } while (0)
 
+// Waits for both of the context's streams (before a buffer is freed).
+inline void syncStreams(Ctx* c) {
+  (void)hipStreamSynchronize(c->stream);
+  if (c->sortStream) (void)hipStreamSynchronize(c->sortStream);
+}
+
 template <typename T> int ensure(Ctx* c, DevBuf<T>& b, size_t count, const char* what) {
   if (count <= b.cap) return LF_OK;
   if (c->capturing)  // a graph would keep the old address
     return fail(c, LF_ERR_STATE, std::string(what) + ": buffer growth during graph capture");
+  syncStreams(c);
+  c->stepInputsKept = false;
   if (b.p) {
     hipError_t e = hipFree(b.p);
     if (e != hipSuccess) return hipFail(c, e, what);
@@ -589,6 +611,8 @@ int ensureKeep(Ctx* c, DevBuf<T>& b, size_t count, size_t keep, const char* what
   if (count <= b.cap) return LF_OK;
   if (c->capturing)
     return fail(c, LF_ERR_STATE, std::string(what) + ": buffer growth during graph capture");
+  syncStreams(c);
+  c->stepInputsKept = false;
   dropGraph(c);
   const size_t want = count + count / 4 + 256;
   T* p = nullptr;
@@ -618,13 +642,23 @@ template <typename T> void release(DevBuf<T>& b) {
 void timingBegin(Ctx* c, const char* name, PendingTiming& t);
 void timingEnd(Ctx* c, PendingTiming& t);
 
+// Kernels that write none of the buffers the key and sort kernels read or
+// write (their outputs are the mesh, the wire staging and the diagnostics).
+inline bool keepsStepInputs(const char* name) {
+  return std::strncmp(name, "mc_", 3) == 0 || std::strncmp(name, "wire_", 5) == 0 ||
+         std::strcmp(name, "chunk_sums") == 0 || std::strcmp(name, "scene_sums") == 0 ||
+         std::strcmp(name, "copy_probe") == 0;
+}
+
 template <typename... KArgs, typename... Args>
 int launch(Ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
            size_t shmem, Args... args) {
   PendingTiming t{};
   const bool timed = c->prof && (c->profOnly.empty() || c->profOnly == name);
+  const hipStream_t stream = c->launchStream ? c->launchStream : c->stream;
+  if (!c->launchStream && !keepsStepInputs(name)) c->stepInputsKept = false;
   if (timed) timingBegin(c, name, t);
-  hipLaunchKernelGGL(kernel, grid, block, shmem, c->stream, args...);
+  hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hipFail(c, e, name);
   if (timed) timingEnd(c, t);

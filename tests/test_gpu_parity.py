@@ -176,6 +176,45 @@ def test_far_neighbours_fall_back_bit_exact(ctx):
     del o
 
 
+def test_steps_with_marching_cubes_between_bit_exact(ctx):
+    """Marching cubes between steps, as a renderer runs them: the next step's
+    key and sort kernels then run on a second stream, overlapping the main
+    stream's marching cubes. Four steps issued without a host read between
+    them, then new wells (which must keep the next step's keys after the
+    upload) and two more: the state and the last density field equal the
+    oracle's bit for bit."""
+    sc = scenes.dam_break_8k()
+    p = sc.params()
+    o = _pair(ctx, sc, p)
+    m = sc.mc()
+
+    def step_and_mesh():
+        ctx.step(p, 1)
+        ctx.mc_sample(p, m)
+        ctx.mc_classify_count(m)
+        ctx.mc_emit(p, m)
+        o.step(1)
+
+    for _ in range(4):
+        step_and_mesh()
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position after 4")
+    wells = [lf.WellSpec(centre=(1.0, 0.5, 0.5), force=40.0)]
+    ctx.set_wells(wells)
+    o.set_wells(wells)
+    try:
+        for _ in range(2):
+            step_and_mesh()
+    finally:
+        ctx.set_wells([])  # the context is shared by the module's tests
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position after 6")
+    _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity after 6")
+    field = ctx.read(LF_ARR_FIELD)
+    ofield = o.mc_density(m.samples)
+    _eq(field.astype(np.float64).reshape(ofield.shape), ofield, "field after 6")
+
+
 def test_density_mc_bit_exact(ctx):
     sc = scenes.dam_break_8k()
     p = sc.params()
