@@ -12,7 +12,10 @@
  * Host pointers are host memory. Everything the step works on lives in device
  * memory owned by the context; it persists across calls and grows on demand.
  * Calls on one context are ordered on the context's own HIP stream; calls that
- * hand data back to the host synchronise that stream first.
+ * hand data back to the host synchronise it first. (lf_pbf_step may run its
+ * key and sort kernels on a second stream of the context, ordered by events
+ * after the previous step and before the rest of the step, so they overlap
+ * the marching cubes issued since; lf_sync waits for both.)
  */
 #ifndef LIBFLUID_AMD_H
 #define LIBFLUID_AMD_H
