@@ -376,9 +376,14 @@ int lf_ctx_create(int device, lf_ctx** out) {
   }
   // default priority, like the main stream; timing-free events for ordering
   if (hipStreamCreateWithFlags(&c->sortStream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->mcStream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->evStepEnd, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evSorted, hipEventDisableTiming) != hipSuccess) {
-    if (c->evStepEnd) (void)hipEventDestroy(c->evStepEnd);
+      hipEventCreateWithFlags(&c->evSorted, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evMcIn, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evMcDone, hipEventDisableTiming) != hipSuccess) {
+    for (hipEvent_t e : {c->evStepEnd, c->evSorted, c->evMcIn, c->evMcDone})
+      if (e) (void)hipEventDestroy(e);
+    if (c->mcStream) (void)hipStreamDestroy(c->mcStream);
     if (c->sortStream) (void)hipStreamDestroy(c->sortStream);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -448,6 +453,9 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::dropGraph(c);
   (void)hipEventDestroy(c->evStepEnd);
   (void)hipEventDestroy(c->evSorted);
+  (void)hipEventDestroy(c->evMcIn);
+  (void)hipEventDestroy(c->evMcDone);
+  (void)hipStreamDestroy(c->mcStream);
   (void)hipStreamDestroy(c->sortStream);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -460,6 +468,8 @@ int lf_sync(lf_ctx* c) {
   if (!c) return LF_ERR_ARG;
   LF_HIP(c, hipStreamSynchronize(c->stream));
   LF_HIP(c, hipStreamSynchronize(c->sortStream));
+  LF_HIP(c, hipStreamSynchronize(c->mcStream));
+  c->mcBusy = false;
   return LF_OK;
 }
 
@@ -707,6 +717,7 @@ int lf_mc_emit(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m) {
 
 int lf_mc_count(lf_ctx* c, size_t* ntri) {
   if (!c || !ntri) return LF_ERR_ARG;
+  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
   if (!c->mcCounted) return fail(c, LF_ERR_STATE, "lf_mc_count: no counts");
   uint32_t t = 0;
   LF_HIP(c, hipMemcpyAsync(&t, c->mcTotal.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -719,6 +730,7 @@ int lf_mc_count(lf_ctx* c, size_t* ntri) {
 
 int lf_mc_download(lf_ctx* c, lf_mesh_triangle* out, size_t cap, size_t* ntri) {
   if (!c || !ntri) return LF_ERR_ARG;
+  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
   if (!c->mcEmitted) return fail(c, LF_ERR_STATE, "lf_mc_download: nothing emitted");
   size_t total = 0;
   int rc = lf_mc_count(c, &total);
@@ -762,6 +774,7 @@ int lf_mc_download(lf_ctx* c, lf_mesh_triangle* out, size_t cap, size_t* ntri) {
 int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[3],
                        const float step[3], float scale, const float* field) {
   if (!c || !samples || !origin || !step || !field) return LF_ERR_ARG;
+  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
   for (int k = 0; k < 3; ++k)
     if (samples[k] < 2) return fail(c, LF_ERR_ARG, "lf_mc_field_upload: samples must be >= 2");
   if (!(scale > 0.0f)) return fail(c, LF_ERR_ARG, "lf_mc_field_upload: scale must be > 0");
@@ -785,6 +798,7 @@ int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[
 
 int lf_mc_set_capacity(lf_ctx* c, size_t max_triangles) {
   if (!c || max_triangles == 0) return LF_ERR_ARG;
+  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
   if (max_triangles != c->triCap) {
     lf::release(c->triPos);
     lf::release(c->triNrm);
@@ -856,6 +870,7 @@ int lf_advance(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m, lf_part
 
 int lf_read(lf_ctx* c, int array, void* dst, size_t bytes) {
   if (!c || (!dst && bytes)) return LF_ERR_ARG;
+  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
   LF_HIP(c, hipSetDevice(c->device));
   const void* src = nullptr;
   size_t avail = 0;
@@ -1164,6 +1179,7 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
 
 int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
   if (!c) return LF_ERR_ARG;
+  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
   c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (nsteps == 0 || (nsteps & 1u))
     return fail(c, LF_ERR_ARG, "lf_graph_capture: nsteps must be even and > 0");
@@ -1210,6 +1226,7 @@ int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
 
 int lf_graph_launch(lf_ctx* c, uint32_t times) {
   if (!c) return LF_ERR_ARG;
+  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
   c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (!c->graph) return fail(c, LF_ERR_STATE, "lf_graph_launch: no captured graph");
   LF_HIP(c, hipSetDevice(c->device));

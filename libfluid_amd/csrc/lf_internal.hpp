@@ -462,6 +462,14 @@ struct Ctx {
   bool stepInputsKept = false;
   bool scenesUploaded = false;  // the last prepareScenes copied constants or chunks
   bool overlapSort = false;     // this step's key and sort kernels run on sortStream
+  // Marching-cubes classification, scan and emission run on mcStream (after
+  // an event on the main stream that covers the sampled field), beside the
+  // next step, which touches none of their buffers. mcBusy: mcStream has work
+  // the main stream has not yet waited for (mcJoin); every reader of the mesh
+  // outputs and every writer of the field joins first.
+  hipStream_t mcStream = nullptr;
+  hipEvent_t evMcIn = nullptr, evMcDone = nullptr;
+  bool mcBusy = false;
   std::string err;
 
   size_t n = 0;
@@ -584,6 +592,15 @@ int hipFail(Ctx* c, hipError_t e, const char* where);
 inline void syncStreams(Ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   if (c->sortStream) (void)hipStreamSynchronize(c->sortStream);
+  if (c->mcStream) (void)hipStreamSynchronize(c->mcStream);
+  c->mcBusy = false;
+}
+
+// The main stream waits for the marching cubes on mcStream.
+inline int mcJoin(Ctx* c) {
+  if (!c->mcBusy) return 0;
+  c->mcBusy = false;
+  return hipStreamWaitEvent(c->stream, c->evMcDone, 0) == hipSuccess ? 0 : LF_ERR_HIP;
 }
 
 template <typename T> int ensure(Ctx* c, DevBuf<T>& b, size_t count, const char* what) {

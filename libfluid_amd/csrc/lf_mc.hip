@@ -627,6 +627,8 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* _
 
 int mcSample(Ctx* c, const lf_mc_params& m) {
   if (!c->gridValid) return fail(c, LF_ERR_STATE, "lf_mc_sample: no grid (call lf_grid_build or lf_pbf_step)");
+  // the field is rewritten: the last classification and emission read it first
+  if (mcJoin(c)) return fail(c, LF_ERR_HIP, "lf_mc_sample: stream wait");
   // the selected scene of a batch: its constants and its slice of the table
   const PbfConsts k = (c->batch && size_t(c->mcScene) < c->sceneConsts.size())
                           ? c->sceneConsts[c->mcScene]
@@ -700,9 +702,35 @@ static CubeGrid cubeGrid(const Ctx* c) {
   return g;
 }
 
+// Scope in which launches go to the context's marching-cubes stream, after
+// everything issued so far on the main stream (the sampled field); outside a
+// graph capture only.
+struct OnMcStream {
+  Ctx* c;
+  bool on;
+  explicit OnMcStream(Ctx* ctx) : c(ctx), on(!ctx->capturing && ctx->mcStream) {
+    if (!on) return;
+    if (hipEventRecord(c->evMcIn, c->stream) != hipSuccess ||
+        hipStreamWaitEvent(c->mcStream, c->evMcIn, 0) != hipSuccess) {
+      on = false;  // stay on the main stream
+      return;
+    }
+    c->launchStream = c->mcStream;
+  }
+  ~OnMcStream() {
+    if (!on) return;
+    c->launchStream = nullptr;
+    if (hipEventRecord(c->evMcDone, c->mcStream) == hipSuccess)
+      c->mcBusy = true;
+    else
+      (void)hipStreamSynchronize(c->mcStream);
+  }
+};
+
 int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
   if (!c->mcSampled) return fail(c, LF_ERR_STATE, "lf_mc_classify_count: no field (call lf_mc_sample)");
   if (!cubesFit(c)) return fail(c, LF_ERR_ARG, "marching cubes: more than 2^31 cubes");
+  OnMcStream onMc(c);
   const CubeGrid g = cubeGrid(c);
   const uint64_t nbl = (uint64_t(g.cubes) + kCubesPerBlock - 1) / kCubesPerBlock;
   int rc = ensure(c, c->mcBlock, size_t(nbl) + 1, "mc block counts");
@@ -726,6 +754,7 @@ int mcEmit(Ctx* c, const lf_mc_params& m) {
   if (!c->mcCounted) return fail(c, LF_ERR_STATE, "lf_mc_emit: no counts (call lf_mc_classify_count)");
   if (!(m.isolevel == c->mcIso))
     return fail(c, LF_ERR_ARG, "lf_mc_emit: isolevel differs from the one lf_mc_classify_count used");
+  OnMcStream onMc(c);
   const CubeGrid g = cubeGrid(c);
   const uint64_t nbl = (uint64_t(g.cubes) + kCubesPerBlock - 1) / kCubesPerBlock;
   int rc = ensure(c, c->triPos, c->triCap * 9, "triangle positions");
