@@ -1125,6 +1125,7 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
              hipOk(hipMemcpyAsync(id0, c->ID[cur0].p, n * 4, hipMemcpyDeviceToDevice, c->stream), "autotune");
     }
     c->cur = cur0;
+    c->stepInputsKept = false;  // the state is rewritten below
     return hipOk(hipMemcpyAsync(c->X[cur0].p, x0, b4, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
            hipOk(hipMemcpyAsync(c->V[cur0].p, v0, b4, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
            hipOk(hipMemcpyAsync(c->T[cur0].p, t0, n, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
