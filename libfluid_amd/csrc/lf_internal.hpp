@@ -31,7 +31,7 @@
 //  * divisions by constants are multiplications by the reciprocal folded on
 //    the host (W6(r) / W6(dq) and / rho0 of sph_delta);
 //  * XSPH takes W6 on d2 (t = h*h - d2, no sqrt); the density field takes
-//    t = fma(-dx,dx, fma(-dy,dy, fma(-dz,dz, h*h))) and (m poly6) t^3.
+//    t = fma(-dx,dx, fma(-dy,dy, fma(-dz,dz, h*h))) and rho = fma((m poly6) t, t t, rho).
 // The FAST variants of the pair kernels (LF_FLAG_FAST_MATH) replace the
 // correctly rounded sqrt and 1/r by the hardware approximations (1 ulp);
 // they are checked against the oracle with a tolerance instead.
