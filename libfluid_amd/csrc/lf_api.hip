@@ -380,8 +380,11 @@ int lf_ctx_create(int device, lf_ctx** out) {
       hipEventCreateWithFlags(&c->evStepEnd, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->evSorted, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->evMcIn, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evMcDone, hipEventDisableTiming) != hipSuccess) {
-    for (hipEvent_t e : {c->evStepEnd, c->evSorted, c->evMcIn, c->evMcDone})
+      hipEventCreateWithFlags(&c->evMcDone, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->pairGuard.ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->pairGuardAlt.ev, hipEventDisableTiming) != hipSuccess) {
+    for (hipEvent_t e : {c->evStepEnd, c->evSorted, c->evMcIn, c->evMcDone, c->pairGuard.ev,
+                         c->pairGuardAlt.ev})
       if (e) (void)hipEventDestroy(e);
     if (c->mcStream) (void)hipStreamDestroy(c->mcStream);
     if (c->sortStream) (void)hipStreamDestroy(c->sortStream);
@@ -417,6 +420,8 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->P0);
   lf::release(c->maxDisp);
   lf::release(c->cellDisp);
+  lf::release(c->Palt);
+  lf::release(c->tableAlt);
   lf::release(c->rho);
   lf::release(c->diffused);
   lf::release(c->table);
@@ -455,6 +460,8 @@ int lf_ctx_destroy(lf_ctx* c) {
   (void)hipEventDestroy(c->evSorted);
   (void)hipEventDestroy(c->evMcIn);
   (void)hipEventDestroy(c->evMcDone);
+  (void)hipEventDestroy(c->pairGuard.ev);
+  (void)hipEventDestroy(c->pairGuardAlt.ev);
   (void)hipStreamDestroy(c->mcStream);
   (void)hipStreamDestroy(c->sortStream);
   (void)hipStreamDestroy(c->stream);
@@ -469,7 +476,7 @@ int lf_sync(lf_ctx* c) {
   LF_HIP(c, hipStreamSynchronize(c->stream));
   LF_HIP(c, hipStreamSynchronize(c->sortStream));
   LF_HIP(c, hipStreamSynchronize(c->mcStream));
-  c->mcBusy = false;
+  lf::syncStreams(c);  // clears the pending stream waits
   return LF_OK;
 }
 
@@ -1180,7 +1187,7 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
 
 int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
   if (!c) return LF_ERR_ARG;
-  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
+  lf::syncStreams(c);  // a capture must not wait on work recorded before it
   c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (nsteps == 0 || (nsteps & 1u))
     return fail(c, LF_ERR_ARG, "lf_graph_capture: nsteps must be even and > 0");

@@ -383,6 +383,15 @@ int gridBuild(Ctx* c) {
   int rc;
   const bool overlap = c->overlapSort;
   c->overlapSort = false;
+  // This step writes the other P / table pair; the density field that last
+  // read that pair (on the marching-cubes stream, two steps ago) comes first.
+  std::swap(c->P, c->Palt);
+  std::swap(c->table, c->tableAlt);
+  std::swap(c->pairGuard, c->pairGuardAlt);
+  if (c->pairGuard.pending) {
+    c->pairGuard.pending = false;
+    LF_HIP(c, hipStreamWaitEvent(c->stream, c->pairGuard.ev, 0));
+  }
   if ((rc = prepareScenes(c))) return rc;
   const uint32_t G = c->tableTotal;
   const int nxt = 1 - c->cur;
@@ -395,6 +404,7 @@ int gridBuild(Ctx* c) {
   if ((rc = ensure(c, c->T[nxt], n, "types"))) return rc;
   if ((rc = ensure(c, c->ID[nxt], n, "ids"))) return rc;
   if ((rc = ensure(c, c->P, n, "p*"))) return rc;
+  if ((rc = ensure(c, c->Palt, n, "p*"))) return rc;  // both pairs: no growth in a graph
   if ((rc = ensure(c, c->PL, n, "p*+lambda"))) return rc;
   if ((rc = ensure(c, c->localKey, n, "cell keys"))) return rc;
   if ((rc = ensure(c, c->nbr, nbrCapacity(n), "neighbour lists"))) return rc;
@@ -404,6 +414,7 @@ int gridBuild(Ctx* c) {
   if ((rc = ensure(c, c->rho, n, "density"))) return rc;
   if ((rc = ensure(c, c->diffused, n, "diffused"))) return rc;
   if ((rc = ensure(c, c->table, size_t(G), "cell table"))) return rc;
+  if ((rc = ensure(c, c->tableAlt, size_t(G), "cell table"))) return rc;
   if ((rc = ensure(c, c->cellDisp, size_t(G), "cell displacement"))) return rc;
   if (n == 0) return LF_OK;
   const Work W = workOf(c, -1);

@@ -487,6 +487,15 @@ struct Ctx {
   DevBuf<uint32_t> cellDisp;  // per cell: float bits of its particles' largest one
   DevBuf<uint32_t> diffused;
   DevBuf<uint32_t> table;
+  // P and the table alternate between steps (gridBuild swaps them): a density
+  // field of the last step may still read the other pair on mcStream. Each
+  // pair carries the event recorded after the last such read (swapped with it).
+  DevBuf<float4> Palt;
+  DevBuf<uint32_t> tableAlt;
+  struct PairGuard {
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  } pairGuard, pairGuardAlt;
   DevBuf<uint32_t> sortHist;
   DevBuf<lf_well> wells;
   int nWells = 0;
@@ -593,7 +602,8 @@ inline void syncStreams(Ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   if (c->sortStream) (void)hipStreamSynchronize(c->sortStream);
   if (c->mcStream) (void)hipStreamSynchronize(c->mcStream);
-  c->mcBusy = false;
+  c->mcBusy = false;  // every event recorded so far has completed
+  c->pairGuard.pending = c->pairGuardAlt.pending = false;
 }
 
 // The main stream waits for the marching cubes on mcStream.

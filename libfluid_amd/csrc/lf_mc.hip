@@ -628,10 +628,33 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* _
 
 }  // namespace
 
+// Scope in which launches go to the context's marching-cubes stream, after
+// everything issued so far on the main stream (the sampled field); outside a
+// graph capture only.
+struct OnMcStream {
+  Ctx* c;
+  bool on;
+  explicit OnMcStream(Ctx* ctx) : c(ctx), on(!ctx->capturing && ctx->mcStream) {
+    if (!on) return;
+    if (hipEventRecord(c->evMcIn, c->stream) != hipSuccess ||
+        hipStreamWaitEvent(c->mcStream, c->evMcIn, 0) != hipSuccess) {
+      on = false;  // stay on the main stream
+      return;
+    }
+    c->launchStream = c->mcStream;
+  }
+  ~OnMcStream() {
+    if (!on) return;
+    c->launchStream = nullptr;
+    if (hipEventRecord(c->evMcDone, c->mcStream) == hipSuccess)
+      c->mcBusy = true;
+    else
+      (void)hipStreamSynchronize(c->mcStream);
+  }
+};
+
 int mcSample(Ctx* c, const lf_mc_params& m) {
   if (!c->gridValid) return fail(c, LF_ERR_STATE, "lf_mc_sample: no grid (call lf_grid_build or lf_pbf_step)");
-  // the field is rewritten: the last classification and emission read it first
-  if (mcJoin(c)) return fail(c, LF_ERR_HIP, "lf_mc_sample: stream wait");
   // the selected scene of a batch: its constants and its slice of the table
   const PbfConsts k = (c->batch && size_t(c->mcScene) < c->sceneConsts.size())
                           ? c->sceneConsts[c->mcScene]
@@ -667,6 +690,9 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
   }
   const unsigned nb = unsigned((total + kThreads - 1) / kThreads);
   if (m.field == LF_MC_FIELD_REFERENCE) {
+    // on the main stream (it reads the state, which the next step rewrites),
+    // after the last classification and emission have read the field
+    if (mcJoin(c)) return fail(c, LF_ERR_HIP, "lf_mc_sample: stream wait");
     if ((rc = ensure(c, c->mcNormal, size_t(total), "mc normals"))) return rc;
     if ((rc = ensure(c, c->mcColour, size_t(total), "mc colours"))) return rc;
     rc = launch(c, "mc_reference_field", k_mc_reference, dim3(nb), dim3(kThreads), 0, total, L, k,
@@ -679,8 +705,17 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
                    tZ = (L.S[2] + kTZ - 1) / kTZ;
     const uint64_t tiles = uint64_t(tX) * tY * tZ;
     if (tiles >= (uint64_t(1) << 31)) return fail(c, LF_ERR_ARG, "lf_mc_sample: lattice too large");
+    // on the marching-cubes stream, after the step: it reads only P and the
+    // table, which the next step leaves alone (it writes the other pair)
+    OnMcStream onMc(c);
     rc = launch(c, "mc_density_field", k_mc_density, dim3(unsigned(tiles)), dim3(kThreads), 0, L,
                 k, table, (const float4*)c->P.p, c->field.p, tY, tZ);
+    if (rc == LF_OK && onMc.on) {
+      if (hipEventRecord(c->pairGuard.ev, c->mcStream) == hipSuccess)
+        c->pairGuard.pending = true;
+      else
+        (void)hipStreamSynchronize(c->mcStream);
+    }
     c->mcAttr = 0;
   }
   if (rc) return rc;
@@ -704,31 +739,6 @@ static CubeGrid cubeGrid(const Ctx* c) {
   g.cubes = uint32_t(uint64_t(g.M[0]) * g.M[1] * g.M[2]);
   return g;
 }
-
-// Scope in which launches go to the context's marching-cubes stream, after
-// everything issued so far on the main stream (the sampled field); outside a
-// graph capture only.
-struct OnMcStream {
-  Ctx* c;
-  bool on;
-  explicit OnMcStream(Ctx* ctx) : c(ctx), on(!ctx->capturing && ctx->mcStream) {
-    if (!on) return;
-    if (hipEventRecord(c->evMcIn, c->stream) != hipSuccess ||
-        hipStreamWaitEvent(c->mcStream, c->evMcIn, 0) != hipSuccess) {
-      on = false;  // stay on the main stream
-      return;
-    }
-    c->launchStream = c->mcStream;
-  }
-  ~OnMcStream() {
-    if (!on) return;
-    c->launchStream = nullptr;
-    if (hipEventRecord(c->evMcDone, c->mcStream) == hipSuccess)
-      c->mcBusy = true;
-    else
-      (void)hipStreamSynchronize(c->mcStream);
-  }
-};
 
 int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
   if (!c->mcSampled) return fail(c, LF_ERR_STATE, "lf_mc_classify_count: no field (call lf_mc_sample)");
