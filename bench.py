@@ -69,12 +69,10 @@ KERNEL_SYMBOL = {
     "pbf_diffuse": "k_diffuse", "mc_density_field": "k_mc_density", "mc_classify_count": "k_mc_count",
     "mc_scan": "k_mc_scan", "mc_emit": "k_mc_emit", "copy_probe": "k_copy_probe",
 }
-# The context stream each launch runs on in a step with marching cubes: the
-# key and sort kernels on the side stream, the marching cubes on their own,
-# the rest on the main stream.
-KERNEL_STREAM = {"predict_key": "side", "radix_hist": "side", "radix_scan": "side",
-                 "radix_scatter": "side", "mc_density_field": "mc", "mc_classify_count": "mc",
-                 "mc_scan": "mc", "mc_emit": "mc"}
+# The context stream each launch runs on: the density-field marching cubes on
+# their own stream (beside the next step), the rest on the main stream.
+KERNEL_STREAM = {"mc_density_field": "mc", "mc_classify_count": "mc", "mc_scan": "mc",
+                 "mc_emit": "mc"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_dam_break_1m.txt")
 
 
@@ -320,9 +318,9 @@ def main():
     ctx.sync()
     stats = ctx.kernel_stats()
     # the dominant kernel: the largest total time on the main stream, the
-    # step's critical path (the key/sort kernels and the marching cubes run on
-    # the context's other streams beside it, and their event-timed durations
-    # include the time they share the GPU)
+    # step's critical path (the marching cubes run on the context's other
+    # stream beside it, and their event-timed durations include the time they
+    # share the GPU)
     main = {k: v for k, v in stats.items() if KERNEL_STREAM.get(k, "main") == "main"}
     dom_name = max(main.items(), key=lambda kv: kv[1][1])[0] if main else None
     print("bench: profiled pass done, dominant kernel %s" % dom_name, file=sys.stderr, flush=True)

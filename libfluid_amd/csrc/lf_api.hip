@@ -142,7 +142,6 @@ static int setParams(Ctx* c, const lf_pbf_params* p) {
 // keyBase_s + G_s] with out-of-table keys clamped to G_s. Uploads happen only
 // when something changed, never during graph capture.
 int prepareScenes(Ctx* c) {
-  c->scenesUploaded = false;
   std::vector<PbfConsts> consts;
   std::vector<uint32_t> begin;
   uint64_t total = 0;
@@ -182,7 +181,6 @@ int prepareScenes(Ctx* c) {
     return fail(c, LF_ERR_STATE, "parameters or sizes changed during graph capture");
   int rc;
   dropGraph(c);
-  c->scenesUploaded = true;
   if (!sameConsts) {
     if ((rc = ensure(c, c->dConsts, consts.size(), "scene constants"))) return rc;
     LF_HIP(c, hipMemcpyAsync(c->dConsts.p, consts.data(), consts.size() * sizeof(PbfConsts),
@@ -300,13 +298,8 @@ static int appendSources(Ctx* c) {
 
 static int pbfStepOnce(Ctx* c) {
   int rc;
-  // the key and sort kernels may overlap the main stream's work since the
-  // last step (its marching cubes) when nothing since then wrote their inputs
-  const bool overlap = c->stepInputsKept && !c->capturing && c->sources.empty() &&
-                       c->hostDrains.empty() && c->nQueries == 0;
   if ((rc = appendSources(c))) return rc;
   if ((rc = applyDrains(c))) return rc;
-  c->overlapSort = overlap;  // consumed by gridBuild
   if ((rc = gridBuild(c))) return rc;
   if (c->nQueries && (rc = answerQueries(c))) return rc;
   if ((rc = pbfBuildLists(c))) return rc;
@@ -324,10 +317,6 @@ static int pbfStepOnce(Ctx* c) {
   if (!fuse && (rc = pbfFinalise(c))) return rc;
   if ((rc = pbfXsph(c))) return rc;
   c->stepDone = true;
-  if (!c->capturing) {
-    LF_HIP(c, hipEventRecord(c->evStepEnd, c->stream));
-    c->stepInputsKept = true;
-  }
   return LF_OK;
 }
 
@@ -374,24 +363,7 @@ int lf_ctx_create(int device, lf_ctx** out) {
     delete c;
     return LF_ERR_HIP;
   }
-  // default priority, like the main stream; timing-free events for ordering
-  if (hipStreamCreateWithFlags(&c->sortStream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->mcStream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evStepEnd, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evSorted, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evMcIn, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evMcDone, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->pairGuard.ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->pairGuardAlt.ev, hipEventDisableTiming) != hipSuccess) {
-    for (hipEvent_t e : {c->evStepEnd, c->evSorted, c->evMcIn, c->evMcDone, c->pairGuard.ev,
-                         c->pairGuardAlt.ev})
-      if (e) (void)hipEventDestroy(e);
-    if (c->mcStream) (void)hipStreamDestroy(c->mcStream);
-    if (c->sortStream) (void)hipStreamDestroy(c->sortStream);
-    (void)hipStreamDestroy(c->stream);
-    delete c;
-    return LF_ERR_HIP;
-  }
+
   *out = c;
   return LF_OK;
 }
@@ -456,14 +428,9 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->sceneDone);
   lf::release(c->scenePasses);
   lf::dropGraph(c);
-  (void)hipEventDestroy(c->evStepEnd);
-  (void)hipEventDestroy(c->evSorted);
-  (void)hipEventDestroy(c->evMcIn);
-  (void)hipEventDestroy(c->evMcDone);
-  (void)hipEventDestroy(c->pairGuard.ev);
-  (void)hipEventDestroy(c->pairGuardAlt.ev);
-  (void)hipStreamDestroy(c->mcStream);
-  (void)hipStreamDestroy(c->sortStream);
+  for (hipEvent_t e : {c->evMcIn, c->evMcDone, c->pairGuard.ev, c->pairGuardAlt.ev})
+    if (e) (void)hipEventDestroy(e);
+  if (c->mcStream) (void)hipStreamDestroy(c->mcStream);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return LF_OK;
@@ -474,8 +441,7 @@ const char* lf_last_error(const lf_ctx* c) { return c ? c->err.c_str() : "null c
 int lf_sync(lf_ctx* c) {
   if (!c) return LF_ERR_ARG;
   LF_HIP(c, hipStreamSynchronize(c->stream));
-  LF_HIP(c, hipStreamSynchronize(c->sortStream));
-  LF_HIP(c, hipStreamSynchronize(c->mcStream));
+  if (c->mcStream) LF_HIP(c, hipStreamSynchronize(c->mcStream));
   lf::syncStreams(c);  // clears the pending stream waits
   return LF_OK;
 }
@@ -483,7 +449,6 @@ int lf_sync(lf_ctx* c) {
 int lf_state_upload(lf_ctx* c, size_t n, const float* pos, const float* vel, const float* mass,
                     const int32_t* type, const uint32_t* colour) {
   if (!c) return LF_ERR_ARG;
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (n > 0 && (!pos || !vel || !mass || !type || !colour))
     return fail(c, LF_ERR_ARG, "lf_state_upload: null array");
   if (n >= (size_t(1) << 28))  // gathers address particles with 32-bit byte offsets
@@ -572,7 +537,6 @@ int lf_state_size(const lf_ctx* c, size_t* n) {
 
 int lf_set_wells(lf_ctx* c, size_t n, const lf_well* wells) {
   if (!c) return LF_ERR_ARG;
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (n > 0 && !wells) return fail(c, LF_ERR_ARG, "lf_set_wells: null wells");
   LF_HIP(c, hipSetDevice(c->device));
   if (n > 0) {
@@ -819,7 +783,6 @@ int lf_advance(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m, lf_part
                size_t n, lf_mesh_triangle* tris, size_t tri_cap, size_t* ntri,
                uint32_t* orderOut) {
   if (!c || !p) return LF_ERR_ARG;
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (ntri) *ntri = 0;
   if (n == 0) return LF_OK;  // oclsph.hpp:711-715: empty input, empty result
   if (!xs) return fail(c, LF_ERR_ARG, "lf_advance: null particles");
@@ -914,7 +877,6 @@ int lf_read(lf_ctx* c, int array, void* dst, size_t bytes) {
 
 int lf_write(lf_ctx* c, int array, const void* src, size_t bytes) {
   if (!c || (!src && bytes)) return LF_ERR_ARG;
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   LF_HIP(c, hipSetDevice(c->device));
   void* dst = nullptr;
   size_t avail = 0;
@@ -1008,7 +970,6 @@ int lf_batch_upload(lf_ctx* c, uint32_t nScenes, const lf_pbf_params* params,
                     const size_t* counts, const float* pos, const float* vel, const float* mass,
                     const int32_t* type, const uint32_t* colour) {
   if (!c) return LF_ERR_ARG;
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (nScenes == 0 || !params || !counts) return fail(c, LF_ERR_ARG, "lf_batch_upload: no scenes");
   size_t n = 0;
   std::vector<uint32_t> begin(nScenes + 1, 0u);
@@ -1074,7 +1035,6 @@ int lf_batch_select(lf_ctx* c, uint32_t scene) {
 
 int lf_exec_config(lf_ctx* c, uint32_t threads, int persistent, float skin) {
   if (!c) return LF_ERR_ARG;
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (threads != 128 && threads != 256 && threads != 512)
     return fail(c, LF_ERR_ARG, "lf_exec_config: threads must be 128, 256 or 512");
   if (!(skin >= 0.0f && skin <= 1.0f))
@@ -1090,7 +1050,6 @@ int lf_exec_config(lf_ctx* c, uint32_t threads, int persistent, float skin) {
 
 int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
   if (!c || steps == 0) return LF_ERR_ARG;
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (c->n == 0 || !(c->params.h > 0.0f))
     return fail(c, LF_ERR_STATE, "lf_autotune: needs a state and one step (for its parameters)");
   LF_HIP(c, hipSetDevice(c->device));
@@ -1132,7 +1091,6 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
              hipOk(hipMemcpyAsync(id0, c->ID[cur0].p, n * 4, hipMemcpyDeviceToDevice, c->stream), "autotune");
     }
     c->cur = cur0;
-    c->stepInputsKept = false;  // the state is rewritten below
     return hipOk(hipMemcpyAsync(c->X[cur0].p, x0, b4, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
            hipOk(hipMemcpyAsync(c->V[cur0].p, v0, b4, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
            hipOk(hipMemcpyAsync(c->T[cur0].p, t0, n, hipMemcpyDeviceToDevice, c->stream), "autotune") &&
@@ -1188,7 +1146,6 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
 int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
   if (!c) return LF_ERR_ARG;
   lf::syncStreams(c);  // a capture must not wait on work recorded before it
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (nsteps == 0 || (nsteps & 1u))
     return fail(c, LF_ERR_ARG, "lf_graph_capture: nsteps must be even and > 0");
   if (!c->stepDone || c->n == 0)
@@ -1235,7 +1192,6 @@ int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
 int lf_graph_launch(lf_ctx* c, uint32_t times) {
   if (!c) return LF_ERR_ARG;
   if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
-  c->stepInputsKept = false;  // writes what the next step's key kernels read
   if (!c->graph) return fail(c, LF_ERR_STATE, "lf_graph_launch: no captured graph");
   LF_HIP(c, hipSetDevice(c->device));
   for (uint32_t t = 0; t < times; ++t) LF_HIP(c, hipGraphLaunch(c->graph, c->stream));

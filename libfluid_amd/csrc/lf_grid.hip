@@ -381,8 +381,6 @@ int copyProbe(Ctx* c, size_t bytes, int reps, float* ms) {
 int gridBuild(Ctx* c) {
   const size_t n = c->n;
   int rc;
-  const bool overlap = c->overlapSort;
-  c->overlapSort = false;
   // This step writes the other P / table pair; the density field that last
   // read that pair (on the marching-cubes stream, two steps ago) comes first.
   std::swap(c->P, c->Palt);
@@ -418,27 +416,12 @@ int gridBuild(Ctx* c) {
   if ((rc = ensure(c, c->cellDisp, size_t(G), "cell displacement"))) return rc;
   if (n == 0) return LF_OK;
   const Work W = workOf(c, -1);
-  // Keys and sort on the side stream, after the end of the last step only,
-  // so they overlap what the main stream still runs of the time since (the
-  // marching cubes of that step); the main stream waits for them before the
-  // reorder. Only when this step uploaded no constants or chunks and nothing
-  // since the last step wrote their inputs (pbfStepOnce).
-  const bool side = overlap && !c->scenesUploaded;
-  if (side) {
-    LF_HIP(c, hipStreamWaitEvent(c->sortStream, c->evStepEnd, 0));
-    c->launchStream = c->sortStream;
-  }
   rc = LF_DISPATCH_BS(c, "predict_key", k_predict_key, W, (const float4*)c->X[c->cur].p,
                       (const float4*)c->V[c->cur].p, (const uint8_t*)c->T[c->cur].p,
                       (const lf_well*)c->wells.p, c->nWells, c->sortKey[0].p, c->sortVal[0].p);
-  int idx = 0;
-  if (rc == LF_OK) rc = radixSort(c, n, idx);
-  c->launchStream = nullptr;
-  if (side) {
-    LF_HIP(c, hipEventRecord(c->evSorted, c->sortStream));
-    LF_HIP(c, hipStreamWaitEvent(c->stream, c->evSorted, 0));
-  }
   if (rc) return rc;
+  int idx = 0;
+  if ((rc = radixSort(c, n, idx))) return rc;
   rc = LF_DISPATCH_BS(c, "reorder", k_reorder, W, uint32_t(n), (const uint32_t*)c->sortVal[idx].p,
                       (const float4*)c->X[c->cur].p, (const float4*)c->V[c->cur].p,
                       (const uint8_t*)c->T[c->cur].p, (const uint32_t*)c->ID[c->cur].p,

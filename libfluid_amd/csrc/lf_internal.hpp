@@ -449,24 +449,14 @@ struct PendingTiming {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  // The next step's key and sort kernels may run on sortStream while the main
-  // stream still runs the marching cubes of the step before (they touch
-  // disjoint buffers). stepInputsKept: since the end of the last step
-  // (evStepEnd), the main stream has received nothing that writes the
-  // buffers those kernels read (state, wells, constants, chunks) or the sort
-  // buffers; kernels other than marching cubes, wire packing and diagnostics
-  // clear it, and so do the entry points that write state or wells.
-  hipStream_t sortStream = nullptr;
-  hipEvent_t evStepEnd = nullptr, evSorted = nullptr;
   hipStream_t launchStream = nullptr;  // non-null: launch() targets this stream
-  bool stepInputsKept = false;
-  bool scenesUploaded = false;  // the last prepareScenes copied constants or chunks
-  bool overlapSort = false;     // this step's key and sort kernels run on sortStream
-  // Marching-cubes classification, scan and emission run on mcStream (after
-  // an event on the main stream that covers the sampled field), beside the
-  // next step, which touches none of their buffers. mcBusy: mcStream has work
-  // the main stream has not yet waited for (mcJoin); every reader of the mesh
-  // outputs and every writer of the field joins first.
+  // The density-field marching cubes (sampling, classification, scan,
+  // emission) run on mcStream, created on first use, after an event at the
+  // end of the step, beside the next step, which touches none of their
+  // buffers (P and the table alternate between steps, see pairGuard).
+  // mcBusy: mcStream has work the main stream has not yet waited for
+  // (mcJoin); every reader of the mesh outputs and every writer of the field
+  // on the main stream joins first.
   hipStream_t mcStream = nullptr;
   hipEvent_t evMcIn = nullptr, evMcDone = nullptr;
   bool mcBusy = false;
@@ -600,7 +590,6 @@ int hipFail(Ctx* c, hipError_t e, const char* where);
 // Waits for both of the context's streams (before a buffer is freed).
 inline void syncStreams(Ctx* c) {
   (void)hipStreamSynchronize(c->stream);
-  if (c->sortStream) (void)hipStreamSynchronize(c->sortStream);
   if (c->mcStream) (void)hipStreamSynchronize(c->mcStream);
   c->mcBusy = false;  // every event recorded so far has completed
   c->pairGuard.pending = c->pairGuardAlt.pending = false;
@@ -618,7 +607,6 @@ template <typename T> int ensure(Ctx* c, DevBuf<T>& b, size_t count, const char*
   if (c->capturing)  // a graph would keep the old address
     return fail(c, LF_ERR_STATE, std::string(what) + ": buffer growth during graph capture");
   syncStreams(c);
-  c->stepInputsKept = false;
   if (b.p) {
     hipError_t e = hipFree(b.p);
     if (e != hipSuccess) return hipFail(c, e, what);
@@ -639,7 +627,6 @@ int ensureKeep(Ctx* c, DevBuf<T>& b, size_t count, size_t keep, const char* what
   if (c->capturing)
     return fail(c, LF_ERR_STATE, std::string(what) + ": buffer growth during graph capture");
   syncStreams(c);
-  c->stepInputsKept = false;
   dropGraph(c);
   const size_t want = count + count / 4 + 256;
   T* p = nullptr;
@@ -669,21 +656,12 @@ template <typename T> void release(DevBuf<T>& b) {
 void timingBegin(Ctx* c, const char* name, PendingTiming& t);
 void timingEnd(Ctx* c, PendingTiming& t);
 
-// Kernels that write none of the buffers the key and sort kernels read or
-// write (their outputs are the mesh, the wire staging and the diagnostics).
-inline bool keepsStepInputs(const char* name) {
-  return std::strncmp(name, "mc_", 3) == 0 || std::strncmp(name, "wire_", 5) == 0 ||
-         std::strcmp(name, "chunk_sums") == 0 || std::strcmp(name, "scene_sums") == 0 ||
-         std::strcmp(name, "copy_probe") == 0;
-}
-
 template <typename... KArgs, typename... Args>
 int launch(Ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
            size_t shmem, Args... args) {
   PendingTiming t{};
   const bool timed = c->prof && (c->profOnly.empty() || c->profOnly == name);
   const hipStream_t stream = c->launchStream ? c->launchStream : c->stream;
-  if (!c->launchStream && !keepsStepInputs(name)) c->stepInputsKept = false;
   if (timed) timingBegin(c, name, t);
   hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
   hipError_t e = hipGetLastError();

@@ -628,13 +628,36 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* _
 
 }  // namespace
 
+// The marching-cubes stream (default priority, like the main stream) and its
+// events, created on first use: a context that never meshes keeps one stream.
+bool mcStreamReady(Ctx* c) {
+  if (c->mcStream) return true;
+  hipStream_t s = nullptr;
+  hipEvent_t e[4] = {};
+  bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+  for (int i = 0; ok && i < 4; ++i)
+    ok = hipEventCreateWithFlags(&e[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+    if (s) (void)hipStreamDestroy(s);
+    return false;
+  }
+  c->mcStream = s;
+  c->evMcIn = e[0];
+  c->evMcDone = e[1];
+  c->pairGuard.ev = e[2];
+  c->pairGuardAlt.ev = e[3];
+  return true;
+}
+
 // Scope in which launches go to the context's marching-cubes stream, after
-// everything issued so far on the main stream (the sampled field); outside a
-// graph capture only.
+// everything issued so far on the main stream; outside a graph capture only
+// (else, or if the stream cannot be made, they stay on the main stream).
 struct OnMcStream {
   Ctx* c;
   bool on;
-  explicit OnMcStream(Ctx* ctx) : c(ctx), on(!ctx->capturing && ctx->mcStream) {
+  explicit OnMcStream(Ctx* ctx) : c(ctx), on(!ctx->capturing && mcStreamReady(ctx)) {
     if (!on) return;
     if (hipEventRecord(c->evMcIn, c->stream) != hipSuccess ||
         hipStreamWaitEvent(c->mcStream, c->evMcIn, 0) != hipSuccess) {

@@ -177,12 +177,11 @@ def test_far_neighbours_fall_back_bit_exact(ctx):
 
 
 def test_steps_with_marching_cubes_between_bit_exact(ctx):
-    """Marching cubes between steps, as a renderer runs them: the next step's
-    key and sort kernels then run on a second stream, overlapping the main
-    stream's marching cubes. Four steps issued without a host read between
-    them, then new wells (which must keep the next step's keys after the
-    upload) and two more: the state and the last density field equal the
-    oracle's bit for bit."""
+    """Marching cubes between steps, as a renderer runs them: they run on a
+    second stream of the context beside the next step, which writes the other
+    of two alternating p* / cell-table buffers. Four steps issued without a
+    host read between them, then new wells and two more: the state and the
+    last density field equal the oracle's bit for bit."""
     sc = scenes.dam_break_8k()
     p = sc.params()
     o = _pair(ctx, sc, p)
