@@ -12,14 +12,12 @@
  * Host pointers are host memory. Everything the step works on lives in device
  * memory owned by the context; it persists across calls and grows on demand.
  * Calls on one context are ordered on the context's own HIP stream; calls that
- * hand data back to the host synchronise it first. (lf_pbf_step may run its
- * key and sort kernels on a second stream of the context, ordered by events
- * after the previous step and before the rest of the step, so they overlap
- * the marching cubes issued since; the density-field lf_mc_sample,
- * lf_mc_classify_count and lf_mc_emit run on a third, after the step, so they
- * overlap the next step, which writes the other of two alternating p* / cell
- * table buffers; every call that reads the mesh or rewrites the field waits
- * for them; lf_sync waits for all three.)
+ * hand data back to the host synchronise it first. (The density-field
+ * lf_mc_sample, lf_mc_classify_count and lf_mc_emit run on a second stream of
+ * the context, created on first use and ordered by events after everything
+ * issued before them, so they overlap the next step, which writes the other
+ * of two alternating p* / cell-table buffers; every call that reads the mesh
+ * or rewrites the field waits for them; lf_sync waits for both.)
  */
 #ifndef LIBFLUID_AMD_H
 #define LIBFLUID_AMD_H
