@@ -630,7 +630,7 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* _
 
 // The marching-cubes stream (default priority, like the main stream) and its
 // events, created on first use: a context that never meshes keeps one stream.
-bool mcStreamReady(Ctx* c) {
+static bool mcStreamReady(Ctx* c) {
   if (c->mcStream) return true;
   hipStream_t s = nullptr;
   hipEvent_t e[4] = {};
