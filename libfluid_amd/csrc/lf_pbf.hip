@@ -114,9 +114,9 @@ template <typename T> __device__ inline T at(const T* __restrict__ base, uint32_
 // Neighbour lists of the predicted p* (see the file comment). P0 (the
 // clamped p*) was written by the reorder kernel. Each lane collects its
 // accepted candidates in a 16-entry LDS ring (two 8-entry groups) and, once
-// per block of candidates, stores a completed group as 32 bytes of its list:
-// a block adds fewer than 8 entries, so at most one group completes in it and
-// the group still to be stored is never overwritten. (Checking after every
+// per block of candidates, stores a completed group as 24 bytes of its list:
+// a block adds at most 8 entries, so at most one group completes in it and the
+// group still to be stored is never overwritten. (Checking after every
 // candidate ran the store path on almost every step: with 64 lanes, some lane
 // nearly always completes a group.)
 template <int BS>
@@ -125,7 +125,9 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
                                                    const uint8_t* __restrict__ T, Lists L,
                                                    int checkFar) {
   constexpr uint32_t kRing = 2 * kNbrGroup;
-  static_assert(kBuildBlock < int(kNbrGroup), "a block completes at most one group");
+  // at most 7 entries wait before a block, so at most 15 after it: one group
+  // completes at most, and a block's writes always find a free slot
+  static_assert(kBuildBlock <= int(kNbrGroup), "a block completes at most one group");
   // win[j * BS + lane] holds entry j mod 16 of the lane's list; lanes side by
   // side, so no bank conflicts
   __shared__ uint32_t win[kRing * BS];
