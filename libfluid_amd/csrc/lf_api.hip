@@ -428,7 +428,7 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->sceneDone);
   lf::release(c->scenePasses);
   lf::dropGraph(c);
-  for (hipEvent_t e : {c->evMcIn, c->evMcDone, c->pairGuard.ev, c->pairGuardAlt.ev})
+  for (hipEvent_t e : {c->evMcIn, c->evMcDone, c->pairGuard.ev, c->pairGuardAlt.ev, c->evPosFinal})
     if (e) (void)hipEventDestroy(e);
   if (c->mcStream) (void)hipStreamDestroy(c->mcStream);
   (void)hipStreamDestroy(c->stream);
@@ -449,6 +449,7 @@ int lf_sync(lf_ctx* c) {
 int lf_state_upload(lf_ctx* c, size_t n, const float* pos, const float* vel, const float* mass,
                     const int32_t* type, const uint32_t* colour) {
   if (!c) return LF_ERR_ARG;
+  c->posEventValid = false;  // P and the table are rewritten outside the event
   if (n > 0 && (!pos || !vel || !mass || !type || !colour))
     return fail(c, LF_ERR_ARG, "lf_state_upload: null array");
   if (n >= (size_t(1) << 28))  // gathers address particles with 32-bit byte offsets
@@ -754,6 +755,7 @@ int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[
   int rc = lf::ensure(c, c->field, total, "mc field");
   if (rc) return rc;
   LF_HIP(c, hipMemcpyAsync(c->field.p, field, total * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  c->fieldOnMain = true;
   LF_HIP(c, hipStreamSynchronize(c->stream));
   for (int k = 0; k < 3; ++k) {
     c->S[k] = samples[k];
@@ -877,6 +879,11 @@ int lf_read(lf_ctx* c, int array, void* dst, size_t bytes) {
 
 int lf_write(lf_ctx* c, int array, const void* src, size_t bytes) {
   if (!c || (!src && bytes)) return LF_ERR_ARG;
+  // the marching-cubes stream may read what is written; the early position
+  // event no longer covers P; a written field is the main stream's
+  if (lf::mcJoin(c)) return LF_ERR_HIP;
+  c->posEventValid = false;
+  if (array == LF_ARR_FIELD) c->fieldOnMain = true;
   LF_HIP(c, hipSetDevice(c->device));
   void* dst = nullptr;
   size_t avail = 0;
@@ -970,6 +977,7 @@ int lf_batch_upload(lf_ctx* c, uint32_t nScenes, const lf_pbf_params* params,
                     const size_t* counts, const float* pos, const float* vel, const float* mass,
                     const int32_t* type, const uint32_t* colour) {
   if (!c) return LF_ERR_ARG;
+  c->posEventValid = false;  // P and the table are rewritten outside the event
   if (nScenes == 0 || !params || !counts) return fail(c, LF_ERR_ARG, "lf_batch_upload: no scenes");
   size_t n = 0;
   std::vector<uint32_t> begin(nScenes + 1, 0u);
@@ -1050,6 +1058,7 @@ int lf_exec_config(lf_ctx* c, uint32_t threads, int persistent, float skin) {
 
 int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
   if (!c || steps == 0) return LF_ERR_ARG;
+  c->posEventValid = false;  // P and the table are rewritten outside the event
   if (c->n == 0 || !(c->params.h > 0.0f))
     return fail(c, LF_ERR_STATE, "lf_autotune: needs a state and one step (for its parameters)");
   LF_HIP(c, hipSetDevice(c->device));
@@ -1145,6 +1154,7 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
 
 int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
   if (!c) return LF_ERR_ARG;
+  c->posEventValid = false;  // P and the table are rewritten outside the event
   lf::syncStreams(c);  // a capture must not wait on work recorded before it
   if (nsteps == 0 || (nsteps & 1u))
     return fail(c, LF_ERR_ARG, "lf_graph_capture: nsteps must be even and > 0");
@@ -1191,6 +1201,7 @@ int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
 
 int lf_graph_launch(lf_ctx* c, uint32_t times) {
   if (!c) return LF_ERR_ARG;
+  c->posEventValid = false;  // P and the table are rewritten outside the event
   if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
   if (!c->graph) return fail(c, LF_ERR_STATE, "lf_graph_launch: no captured graph");
   LF_HIP(c, hipSetDevice(c->device));

@@ -431,6 +431,7 @@ int gridBuild(Ctx* c) {
   rc = launch(c, "cell_table", k_cell_table, dim3(blocksFor(G, kCells)), dim3(kThreads), 0,
               (const uint32_t*)c->sortKey[idx].p, uint32_t(n), G, c->table.p, c->cellDisp.p);
   if (rc) return rc;
+  notePositionsWritten(c);  // P (reorder) and the table
   c->keysIdx = idx;
   c->gridG = c->consts.G;
   c->gridTotal = G;

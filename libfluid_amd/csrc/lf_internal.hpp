@@ -460,6 +460,15 @@ struct Ctx {
   hipStream_t mcStream = nullptr;
   hipEvent_t evMcIn = nullptr, evMcDone = nullptr;
   bool mcBusy = false;
+  // evPosFinal: recorded on the main stream after every kernel that writes P
+  // or the table (grid build, delta, finalise), so a density field waits for
+  // the last of them rather than for everything issued since (XSPH);
+  // posEventValid: no other write of P or the table came after it.
+  // fieldOnMain: the field was last written on the main stream (an upload or
+  // the metaball sampling), so classification waits for the main stream.
+  hipEvent_t evPosFinal = nullptr;
+  bool posEventValid = false;
+  bool fieldOnMain = false;
   std::string err;
 
   size_t n = 0;
@@ -593,6 +602,12 @@ inline void syncStreams(Ctx* c) {
   if (c->mcStream) (void)hipStreamSynchronize(c->mcStream);
   c->mcBusy = false;  // every event recorded so far has completed
   c->pairGuard.pending = c->pairGuardAlt.pending = false;
+}
+
+// After a kernel that writes P or the table on the main stream.
+inline void notePositionsWritten(Ctx* c) {
+  c->posEventValid = c->evPosFinal && !c->capturing &&
+                     hipEventRecord(c->evPosFinal, c->stream) == hipSuccess;
 }
 
 // The main stream waits for the marching cubes on mcStream.

@@ -633,9 +633,9 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* _
 static bool mcStreamReady(Ctx* c) {
   if (c->mcStream) return true;
   hipStream_t s = nullptr;
-  hipEvent_t e[4] = {};
+  hipEvent_t e[5] = {};
   bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
-  for (int i = 0; ok && i < 4; ++i)
+  for (int i = 0; ok && i < 5; ++i)
     ok = hipEventCreateWithFlags(&e[i], hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     for (hipEvent_t x : e)
@@ -648,22 +648,35 @@ static bool mcStreamReady(Ctx* c) {
   c->evMcDone = e[1];
   c->pairGuard.ev = e[2];
   c->pairGuardAlt.ev = e[3];
+  c->evPosFinal = e[4];
+  c->posEventValid = false;  // nothing recorded yet
   return true;
 }
 
-// Scope in which launches go to the context's marching-cubes stream, after
-// everything issued so far on the main stream; outside a graph capture only
-// (else, or if the stream cannot be made, they stay on the main stream).
+// Scope in which launches go to the context's marching-cubes stream; outside
+// a graph capture only (else, or if the stream cannot be made, they stay on
+// the main stream). It first waits on the main stream for what the launches
+// read there: the last write of P and the table (sampling), or, when the
+// field was last written on the main stream, everything issued so far
+// (classification, emission); else only for its own earlier work.
+enum class McInput { Positions, Field };
 struct OnMcStream {
   Ctx* c;
   bool on;
-  explicit OnMcStream(Ctx* ctx) : c(ctx), on(!ctx->capturing && mcStreamReady(ctx)) {
+  OnMcStream(Ctx* ctx, McInput in) : c(ctx), on(!ctx->capturing && mcStreamReady(ctx)) {
     if (!on) return;
-    if (hipEventRecord(c->evMcIn, c->stream) != hipSuccess ||
-        hipStreamWaitEvent(c->mcStream, c->evMcIn, 0) != hipSuccess) {
+    hipError_t e = hipSuccess;
+    if (in == McInput::Positions && c->posEventValid) {
+      e = hipStreamWaitEvent(c->mcStream, c->evPosFinal, 0);
+    } else if (in == McInput::Positions || c->fieldOnMain) {
+      e = hipEventRecord(c->evMcIn, c->stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(c->mcStream, c->evMcIn, 0);
+    }
+    if (e != hipSuccess) {
       on = false;  // stay on the main stream
       return;
     }
+    c->fieldOnMain = false;
     c->launchStream = c->mcStream;
   }
   ~OnMcStream() {
@@ -722,6 +735,7 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
                 m.resolution, m.particle_size, m.particle_influence, table,
                 (const uint8_t*)c->T[c->cur].p, (const float4*)c->X[c->cur].p,
                 (const float4*)c->V[c->cur].p, c->field.p, c->mcNormal.p, c->mcColour.p);
+    c->fieldOnMain = true;
     c->mcAttr = 1;
   } else {
     const uint32_t tX = (L.S[0] + kTX - 1) / kTX, tY = (L.S[1] + kTY - 1) / kTY,
@@ -730,7 +744,7 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
     if (tiles >= (uint64_t(1) << 31)) return fail(c, LF_ERR_ARG, "lf_mc_sample: lattice too large");
     // on the marching-cubes stream, after the step: it reads only P and the
     // table, which the next step leaves alone (it writes the other pair)
-    OnMcStream onMc(c);
+    OnMcStream onMc(c, McInput::Positions);
     rc = launch(c, "mc_density_field", k_mc_density, dim3(unsigned(tiles)), dim3(kThreads), 0, L,
                 k, table, (const float4*)c->P.p, c->field.p, tY, tZ);
     if (rc == LF_OK && onMc.on) {
@@ -766,7 +780,7 @@ static CubeGrid cubeGrid(const Ctx* c) {
 int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
   if (!c->mcSampled) return fail(c, LF_ERR_STATE, "lf_mc_classify_count: no field (call lf_mc_sample)");
   if (!cubesFit(c)) return fail(c, LF_ERR_ARG, "marching cubes: more than 2^31 cubes");
-  OnMcStream onMc(c);
+  OnMcStream onMc(c, McInput::Field);
   const CubeGrid g = cubeGrid(c);
   const uint64_t nbl = (uint64_t(g.cubes) + kCubesPerBlock - 1) / kCubesPerBlock;
   int rc = ensure(c, c->mcBlock, size_t(nbl) + 1, "mc block counts");
@@ -790,7 +804,7 @@ int mcEmit(Ctx* c, const lf_mc_params& m) {
   if (!c->mcCounted) return fail(c, LF_ERR_STATE, "lf_mc_emit: no counts (call lf_mc_classify_count)");
   if (!(m.isolevel == c->mcIso))
     return fail(c, LF_ERR_ARG, "lf_mc_emit: isolevel differs from the one lf_mc_classify_count used");
-  OnMcStream onMc(c);
+  OnMcStream onMc(c, McInput::Field);
   const CubeGrid g = cubeGrid(c);
   const uint64_t nbl = (uint64_t(g.cubes) + kCubesPerBlock - 1) / kCubesPerBlock;
   int rc = ensure(c, c->triPos, c->triCap * 9, "triangle positions");

@@ -738,15 +738,19 @@ int pbfDelta(Ctx* c, bool skipDone, bool final) {
     });
   });
   c->listFresh = false;  // p* now differs from the predicted positions
+  if (rc == LF_OK) notePositionsWritten(c);
   return rc;
 }
 
 int pbfFinalise(Ctx* c) {
   if (c->n == 0) return LF_OK;
   const Work W = workOf(c, -1);
-  return LF_DISPATCH_BS(c, "pbf_finalise", k_finalise, W, (const uint8_t*)c->T[c->cur].p, c->P.p,
-                        c->X[c->cur].p, c->V[c->cur].p, (const uint32_t*)c->diffused.p,
-                        int((c->consts.flags & LF_FLAG_DIFFUSE) != 0));
+  const int rc = LF_DISPATCH_BS(c, "pbf_finalise", k_finalise, W, (const uint8_t*)c->T[c->cur].p,
+                                c->P.p, c->X[c->cur].p, c->V[c->cur].p,
+                                (const uint32_t*)c->diffused.p,
+                                int((c->consts.flags & LF_FLAG_DIFFUSE) != 0));
+  if (rc == LF_OK) notePositionsWritten(c);  // P.w
+  return rc;
 }
 
 int pbfXsph(Ctx* c) {
