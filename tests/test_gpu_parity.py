@@ -464,6 +464,32 @@ def test_mesh_arbitrary_field(ctx):
     _eq(tris["v"], otris["v"], "sphere mesh")
 
 
+def test_field_written_after_density_sampling(ctx):
+    """lf_write of the field right after a density field was sampled on the
+    marching-cubes stream: the write waits for the sampling, and
+    classification and emission (back on that stream) mesh the written
+    field, equal to the oracle's marching cubes of it."""
+    sc = scenes.dam_break_8k()
+    p = sc.params()
+    ctx.upload(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
+    ctx.step(p, 1)
+    m = sc.mc()
+    ctx.mc_sample(p, m)
+    S, origin, step = ctx.mc_info()
+    axes = [np.float32(origin[k]) + np.arange(S[k], dtype=np.float32) * np.float32(step[k])
+            for k in range(3)]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    d = np.sqrt(((X - 1.0) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2).astype(np.float32))
+    field = (np.float32(m.isolevel) * (np.float32(1.5) - d / np.float32(0.25))).astype(np.float32)
+    ctx.write(LF_ARR_FIELD, field)
+    ctx.mc_classify_count(m)
+    ctx.mc_emit(p, m)
+    tris = ctx.mc_triangles()
+    otris = oracle.mc_emit(field.astype(np.float64), m.isolevel, origin, step, p.scale)
+    assert len(tris) == len(otris) > 500
+    _eq(tris["v"], otris["v"], "written field mesh")
+
+
 def _ramp_mesh():
     """Two triangles forming a tilted ramp across the 8k box's floor region,
     plus one far away that never touches (plane beyond 40 units)."""
