@@ -689,7 +689,7 @@ int lf_mc_emit(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m) {
 
 int lf_mc_count(lf_ctx* c, size_t* ntri) {
   if (!c || !ntri) return LF_ERR_ARG;
-  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
+  if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   if (!c->mcCounted) return fail(c, LF_ERR_STATE, "lf_mc_count: no counts");
   uint32_t t = 0;
   LF_HIP(c, hipMemcpyAsync(&t, c->mcTotal.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -702,7 +702,7 @@ int lf_mc_count(lf_ctx* c, size_t* ntri) {
 
 int lf_mc_download(lf_ctx* c, lf_mesh_triangle* out, size_t cap, size_t* ntri) {
   if (!c || !ntri) return LF_ERR_ARG;
-  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
+  if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   if (!c->mcEmitted) return fail(c, LF_ERR_STATE, "lf_mc_download: nothing emitted");
   size_t total = 0;
   int rc = lf_mc_count(c, &total);
@@ -746,7 +746,7 @@ int lf_mc_download(lf_ctx* c, lf_mesh_triangle* out, size_t cap, size_t* ntri) {
 int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[3],
                        const float step[3], float scale, const float* field) {
   if (!c || !samples || !origin || !step || !field) return LF_ERR_ARG;
-  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
+  if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   for (int k = 0; k < 3; ++k)
     if (samples[k] < 2) return fail(c, LF_ERR_ARG, "lf_mc_field_upload: samples must be >= 2");
   if (!(scale > 0.0f)) return fail(c, LF_ERR_ARG, "lf_mc_field_upload: scale must be > 0");
@@ -771,7 +771,7 @@ int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[
 
 int lf_mc_set_capacity(lf_ctx* c, size_t max_triangles) {
   if (!c || max_triangles == 0) return LF_ERR_ARG;
-  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
+  if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   if (max_triangles != c->triCap) {
     lf::release(c->triPos);
     lf::release(c->triNrm);
@@ -842,7 +842,7 @@ int lf_advance(lf_ctx* c, const lf_pbf_params* p, const lf_mc_params* m, lf_part
 
 int lf_read(lf_ctx* c, int array, void* dst, size_t bytes) {
   if (!c || (!dst && bytes)) return LF_ERR_ARG;
-  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
+  if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   LF_HIP(c, hipSetDevice(c->device));
   const void* src = nullptr;
   size_t avail = 0;
@@ -881,7 +881,7 @@ int lf_write(lf_ctx* c, int array, const void* src, size_t bytes) {
   if (!c || (!src && bytes)) return LF_ERR_ARG;
   // the marching-cubes stream may read what is written; the early position
   // event no longer covers P; a written field is the main stream's
-  if (lf::mcJoin(c)) return LF_ERR_HIP;
+  if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   c->posEventValid = false;
   if (array == LF_ARR_FIELD) c->fieldOnMain = true;
   LF_HIP(c, hipSetDevice(c->device));
@@ -1202,7 +1202,7 @@ int lf_graph_capture(lf_ctx* c, uint32_t nsteps, const lf_mc_params* m) {
 int lf_graph_launch(lf_ctx* c, uint32_t times) {
   if (!c) return LF_ERR_ARG;
   c->posEventValid = false;  // P and the table are rewritten outside the event
-  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
+  if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   if (!c->graph) return fail(c, LF_ERR_STATE, "lf_graph_launch: no captured graph");
   LF_HIP(c, hipSetDevice(c->device));
   for (uint32_t t = 0; t < times; ++t) LF_HIP(c, hipGraphLaunch(c->graph, c->stream));

@@ -111,7 +111,7 @@ int lf_wire_particles(lf_ctx* c, void* dst, size_t capacity, int64_t timestamp, 
 
 int lf_wire_triangles(lf_ctx* c, void* dst, size_t capacity, int64_t timestamp, size_t* bytes) {
   if (!c || !bytes) return LF_ERR_ARG;
-  if (lf::mcJoin(c)) return LF_ERR_HIP;  // marching cubes on their stream first
+  if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   if (!c->mcEmitted) return fail(c, LF_ERR_STATE, "lf_wire_triangles: nothing emitted");
   size_t total = 0;
   int rc = lf_mc_count(c, &total);
