@@ -673,7 +673,11 @@ struct OnMcStream {
       if (e == hipSuccess) e = hipStreamWaitEvent(c->mcStream, c->evMcIn, 0);
     }
     if (e != hipSuccess) {
-      on = false;  // stay on the main stream
+      // stay on the main stream, after the earlier work of the marching-cubes
+      // stream (it may still read the field these launches rewrite)
+      on = false;
+      (void)hipStreamSynchronize(c->mcStream);
+      c->mcBusy = false;
       return;
     }
     c->fieldOnMain = false;
