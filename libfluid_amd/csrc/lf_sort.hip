@@ -6,13 +6,14 @@
 //
 // Keys are 30-bit Morton codes (10 bits per axis, curves.h:56-72): three
 // passes of 10-bit digits. Each pass is three kernels over tiles of 2048
-// pairs (256 threads x 8):
-//   hist    per-tile digit counts (wave-aggregated LDS counters), stored
-//           tile-major: one contiguous 4 KB row per tile
+// pairs:
+//   hist    per-tile digit counts (512 threads x 4, wave-aggregated LDS
+//           counters), stored tile-major: one contiguous 4 KB row per tile
 //   scan    per-digit exclusive scan over tiles (16 digits x 64 tile
 //           segments per workgroup)
-//   scatter digit bases + stable in-tile rank (per-wave ballot match, LDS
-//           running counters in (wave, slot, lane) = input order) -> output
+//   scatter digit bases + stable in-tile rank (256 threads x 8, per-wave
+//           ballot match, LDS running counters in (wave, slot, lane) = input
+//           order) -> output
 #include "lf_internal.hpp"
 
 namespace lf {
@@ -43,17 +44,21 @@ __device__ inline uint64_t matchDigit(uint32_t d, uint64_t active) {
   return m;
 }
 
-__global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t* __restrict__ keys,
-                                                         uint32_t n, int shift,
-                                                         uint32_t* __restrict__ hist,
-                                                         uint32_t nTiles) {
+// The histogram kernel covers a tile with 512 threads x 4 keys (14.3 us per
+// pass at 1M against 15.5 with the scatter's 256 x 8).
+constexpr int kHistThreads = 512;
+constexpr int kHistItems = int(kTile) / kHistThreads;
+__global__ __launch_bounds__(kHistThreads) void k_radix_hist(const uint32_t* __restrict__ keys,
+                                                             uint32_t n, int shift,
+                                                             uint32_t* __restrict__ hist,
+                                                             uint32_t nTiles) {
   __shared__ uint32_t h[kRadix];
-  for (uint32_t i = threadIdx.x; i < kRadix; i += kThreads) h[i] = 0;
+  for (uint32_t i = threadIdx.x; i < kRadix; i += kHistThreads) h[i] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * kTile;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const uint32_t e = base + j * kThreads + threadIdx.x;
+  for (int j = 0; j < kHistItems; ++j) {
+    const uint32_t e = base + j * kHistThreads + threadIdx.x;
     const bool valid = e < n;
     const uint32_t d = valid ? (keys[e] >> shift) & (kRadix - 1u) : 0u;
     const uint64_t peers = matchDigit(d, __ballot(valid));
@@ -61,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const uint32_t* __restr
     if (valid && __lane_id() == leader) atomicAdd(&h[d], uint32_t(__popcll(peers)));
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kRadix; i += kThreads) hist[size_t(blockIdx.x) * kRadix + i] = h[i];
+  for (uint32_t i = threadIdx.x; i < kRadix; i += kHistThreads) hist[size_t(blockIdx.x) * kRadix + i] = h[i];
 }
 
 // Block-wide exclusive scan of one value per thread (256 threads); returns
@@ -248,7 +253,7 @@ int radixSort(Ctx* c, size_t n, int& outIdx) {
   int src = 0;
   for (int pass = 0; pass < kPasses; ++pass) {
     const int shift = pass * kDigitBits;
-    rc = launch(c, "radix_hist", k_radix_hist, dim3(nTiles), dim3(kThreads), 0,
+    rc = launch(c, "radix_hist", k_radix_hist, dim3(nTiles), dim3(kHistThreads), 0,
                 (const uint32_t*)c->sortKey[src].p, uint32_t(n), shift, hist, nTiles);
     if (rc) return rc;
     rc = launch(c, "radix_scan", k_radix_scan, dim3(kRadix / kScanDigits), dim3(kScanThreads), 0,
