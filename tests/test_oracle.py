@@ -500,3 +500,56 @@ def test_diffuse_brute_force_over_cells():
         assert np.array_equal(got[s][~edge], want[~edge].astype(np.uint8)), s
         assert np.all(np.abs(got[s][edge].astype(int) - np.round(out[edge])) <= 1)
     assert near_integer < n // 10
+
+
+def test_mc_reference_attributes_brute_force():
+    """mc_lattice's value, normal and colour (oclsph_kernel.h:434-441, 497-529)
+    restated as a brute-force sum over every particle, without cells: sample
+    a = (minExt + (x, y, z) * h / res) * scale; each non-obstacle particle b
+    with |p_b - a| < h * scale adds size / |l|^infl to the value,
+    -infl * size * l / |l|^infl to the normal (l = p_b - a, normalised at the
+    end) and its colour to a sum whose mean, converted toward zero, is the
+    sample's colour. Interior samples only: the reference's clamped cells
+    repeat edge cells there."""
+    from libfluid_amd._abi import LF_ARR_DIFFUSED, LF_OBSTACLE
+    rng = np.random.default_rng(12)
+    n, h = 600, 0.1
+    pos = rng.uniform(0.3, 0.7, (n, 3))
+    colour = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    ptype = np.where(rng.random(n) < 0.15, LF_OBSTACLE, 0).astype(np.int32)
+    p = lf.pbf_params(h, gravity=(0, 0, 0), max_bound=(1, 1, 1))
+    o = oracle.Pbf(p, fp64=True)
+    o.load(pos, np.zeros_like(pos), np.ones(n), ptype, colour)
+    o.grid_build()
+    m = lf.mc_reference_params(isolevel=100.0, resolution=1.0)
+    S, field, normal, col = o.mc_reference(m)
+    order = o.get(LF_ARR_ORDER).astype(np.int64)
+    cs = o.get(LF_ARR_DIFFUSED).astype(np.uint32)
+    assert np.array_equal(cs, colour[order])  # no diffusion pass ran
+    xs, ts = pos[order], ptype[order]
+    cb = cs.view(np.uint8).reshape(n, 4).astype(np.float64)
+    _, ext, mn = o.grid_info()
+    size, infl = float(m.particle_size), float(m.particle_influence)
+    hf = float(np.float32(h))  # SPH_H is a float constant: the lattice step and threshold use it
+    checked = filled = 0
+    for x in range(1, min(S[0], ext[0] - 1)):
+        for y in range(1, min(S[1], ext[1] - 1)):
+            for z in range(1, min(S[2], ext[2] - 1)):
+                a = np.asarray(mn) + np.array([x, y, z], np.float64) * hf
+                l = xs - a
+                ln = np.sqrt((l * l).sum(1))
+                sel = (ln < hf) & (ts != LF_OBSTACLE)
+                den = ln[sel] ** infl
+                v = (size / den).sum()
+                nrm = (-infl * size * l[sel] / den[:, None]).sum(0)
+                if sel.any():
+                    nrm /= np.sqrt((nrm * nrm).sum())
+                    c = np.trunc(cb[sel].sum(0) / sel.sum()).astype(np.uint8)
+                    filled += 1
+                else:
+                    c = np.zeros(4, np.uint8)
+                assert math.isclose(field[x, y, z], v, rel_tol=1e-12, abs_tol=1e-300)
+                assert np.allclose(normal[x, y, z, :3], nrm, rtol=0, atol=1e-12)
+                assert np.array_equal(np.array([col[x, y, z]], np.uint32).view(np.uint8), c)
+                checked += 1
+    assert checked > 1000 and filled > 100
