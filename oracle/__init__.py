@@ -21,15 +21,17 @@ GRID, LAMBDA, DELTA, FINALISE, XSPH, DIFFUSE, COMMIT, STEP = range(8)
 # extra array ids beyond LF_ARR_*
 CALLER_POSITION, CALLER_VELOCITY, CALLER_MASS, CALLER_KEYS, CALLER_COLOUR = 101, 102, 103, 104, 105
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_LIB):
+def lib(path=None):
+    """The oracle library (ORACLE_LIB, or a build of it at `path`: the
+    planted-mistake builds of tests/test_oracle_mutations.py)."""
+    path = path or ORACLE_LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             raise RuntimeError("oracle not built: make -C oracle")
-        L = ctypes.CDLL(ORACLE_LIB)
+        L = ctypes.CDLL(path)
         P = ctypes.c_void_p
         L.oc_create.restype = P
         L.oc_create.argtypes = [P, ctypes.c_int]
@@ -66,8 +68,8 @@ def lib():
                                  P, ctypes.c_size_t]
         L.oc_mc_table.restype = ctypes.c_int
         L.oc_mc_table.argtypes = [ctypes.c_int] * 3
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def ref_lib():
@@ -95,8 +97,8 @@ def _p(a):
 class Pbf:
     """One oracle instance (FP32 reference semantics, or FP64)."""
 
-    def __init__(self, params, fp64=False):
-        self._L = lib()
+    def __init__(self, params, fp64=False, lib_path=None):
+        self._L = lib(lib_path)
         self.fp64 = fp64
         self._h = self._L.oc_create(ctypes.byref(params), 1 if fp64 else 0)
         self.params = params
