@@ -33,24 +33,34 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(name, n, G, S, T):
-    """Compulsory HBM bytes of one launch: each input read once, each output
-    written once; vec3 = 12 B, scalars and keys 4 B, type 1 B.
-    n particles, G cell-table entries, S lattice samples, T triangles."""
+    """Compulsory HBM bytes of one launch, the per-kernel table of the path's
+    measurement definition: each input array read once and each output written
+    once; vec3 = 12 B, scalars and keys 4 B, type 1 B. n particles, G
+    cell-table entries, S lattice samples, T triangles.
+
+    The grid build's kernels (predict_count, the bucket scan, scatter,
+    large-bucket sort, reorder) together replace key + sort + reorder + cell
+    start, priced as a whole at 20 n + 16 n + 88 n + (4 n + 4 G); each kernel's
+    share below is its own compulsory traffic."""
     cubes = S  # (S-1)^3 ~ S^3 at 256^3
     table = {
-        "predict_key": 29 * n + 8 * n,          # x, v, m, type -> key, index
-        "radix_hist": 4 * n,                    # keys
-        "radix_scatter": 8 * n + 8 * n,         # (key, index) in and out
-        "reorder": 4 * n + 33 * n + 4 * n + 33 * n + 12 * n,  # perm, state in/out, p*
-        "cell_table": 4 * n + 4 * G,
-        "pbf_lambda": 21 * n + 4 * n + 4 * G + 4 * n,  # p*, m, type, key, table -> lambda, rho
-        "pbf_lambda_kahan": 21 * n + 4 * n + 4 * G + 4 * n,
-        "pbf_delta": 29 * n + 4 * n + 4 * G,   # p*, lambda, type, key, mass, table -> p*
-        # the last delta pass of a step, finalise fused: + x, v in and out
-        "pbf_delta_final": 29 * n + 4 * n + 4 * G + 48 * n,
-        "pbf_finalise": 1 * n + 12 * n + 24 * n + 24 * n,  # type, p*, x, v -> x, v
-        "pbf_xsph": 12 * n + 1 * n + 12 * n + 4 * n + 4 * G + 12 * n,
-        "pbf_diffuse": 1 * n + 4 * n + 4 * n + 4 * G + 4 * n,
+        "predict_count": 20 * n + 4 * n,        # p* (from x, v) -> key, rank; counts
+        "bucket_reduce": 4 * G,                 # counts
+        "bucket_scan": 4 * G + 4 * G,           # counts -> table
+        "bucket_scatter": 8 * n + 8 * n,        # key, rank -> key, index
+        "reorder": 88 * n,                      # state in order, p* and keys out
+        "pbf_lambda": 21 * n + 4 * G,           # p*, mass/type, key, table -> lambda
+        "pbf_lambda_kahan": 21 * n + 4 * G,
+        "pbf_lambda_fast": 21 * n + 4 * G,
+        "pbf_lambda_kahan_fast": 21 * n + 4 * G,
+        "pbf_delta": 29 * n + 4 * G,            # p*, lambda, key, table -> p*
+        "pbf_delta_fast": 29 * n + 4 * G,
+        # the last delta pass of a step with finalise fused in: + finalise's 60 n
+        "pbf_delta_final": 29 * n + 4 * G + 60 * n,
+        "pbf_delta_final_fast": 29 * n + 4 * G + 60 * n,
+        "pbf_finalise": 60 * n,
+        "pbf_xsph": 37 * n + 4 * G,
+        "pbf_diffuse": 17 * n + 4 * G,
         "mc_density_field": 4 * S + 16 * n + 4 * G,  # field out; particles + table in
         "mc_classify_count": 4 * S + cubes,     # field in, one case byte per cube out
         "mc_emit": cubes + 36 * T,              # case bytes in, triangle positions out
@@ -58,11 +68,21 @@ def algorithmic_bytes(name, n, G, S, T):
     return table.get(name)
 
 
+GRID_BUILD = ("predict_count", "bucket_reduce", "bucket_scan", "bucket_scatter", "sort_big",
+              "reorder")
+
+
+def grid_build_bytes(n, G):
+    """Key + one logical sort permutation + reorder + cell start."""
+    return 20 * n + 16 * n + 88 * n + 4 * n + 4 * G
+
+
 # The kernel behind each launch name of the library: a rocprof name prefix, or
 # a pattern of the full name where template arguments tell launches apart.
 KERNEL_SYMBOL = {
-    "predict_key": "k_predict_key", "radix_hist": "k_radix_hist", "radix_scan": "k_radix_scan",
-    "radix_scatter": "k_radix_scatter", "reorder": "k_reorder", "cell_table": "k_cell_table",
+    "predict_count": "k_predict_count", "bucket_reduce": "k_bucket_reduce",
+"bucket_scan": "k_bucket_scan",
+    "bucket_scatter": "k_bucket_scatter", "sort_big": "k_sort_big", "reorder": "k_reorder",
     "pbf_build_list": "k_build_list", "pbf_lambda": "k_lambda", "pbf_lambda_kahan": "k_lambda",
     "pbf_delta": r"k_delta<\d+, false, false>", "pbf_delta_final": r"k_delta<\d+, false, true>",
     "pbf_finalise": "k_finalise", "pbf_xsph": "k_xsph",
@@ -125,6 +145,58 @@ def summarize_pmc(fetch_dir, write_dir, copy_bytes=1 << 28):
         fb, wb = fetch.get(k, 0.0), write.get(k, 0.0)
         total = int(round((fb * ff + wb * fw) * 1024))
         print("| %s | %.0f | %.0f | %d | fetch x%.3f, write x%.3f |" % (k, fb, wb, total, ff, fw))
+
+
+def load_trace(path):
+    """(name, start_ns, end_ns) of every kernel dispatch of a rocprofv3
+    --kernel-trace run, in start order: the SQLite results database (ROCm 7's
+    default output) or a kernel_trace.csv."""
+    if path.endswith(".csv"):
+        import csv
+        rows = [(r["Kernel_Name"], int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+                for r in csv.DictReader(open(path))]
+    else:
+        import sqlite3
+        db = sqlite3.connect(path)
+        rows = db.execute("select name, start, end from kernels").fetchall()
+    return sorted(rows, key=lambda r: r[1])
+
+
+def summarize_trace(path, steps, warmup, iterations=4):
+    """Per-kernel launch statistics of a rocprofv3 kernel trace of
+    `bench.py --steps K --warmup W --no-cpu --batch-scenes 0 --no-combined`, as a
+    markdown table: over every launch of the process, and over the timed window
+    alone (the K steps after the W warm-up steps of the second pass over the
+    scene, the steps bench.py's value and roofline are measured on). Steps are
+    counted by the launches of k_predict_count (one per step)."""
+    rows = load_trace(path)
+    sym = lambda name: re.sub(r"^void ", "", re.sub(r"\(.*", "", name))  # noqa: E731
+    starts = [r[1] for r in rows if sym(r[0]).startswith("lf::(anonymous namespace)::k_predict_count")
+              or sym(r[0]).startswith("lf::k_predict_count")]
+    per_pass = warmup + steps
+    t0 = starts[per_pass + warmup] if len(starts) > per_pass + warmup else None
+    t1 = starts[2 * per_pass] if len(starts) > 2 * per_pass else (rows[-1][2] + 1)
+
+    def stats(sel):
+        acc = {}
+        for name, a, b in sel:
+            k = sym(name).replace("lf::(anonymous namespace)::", "").replace("lf::", "")
+            acc.setdefault(k, []).append((b - a) / 1000.0)
+        return acc
+
+    every = stats(rows)
+    window = stats([r for r in rows if t0 is not None and t0 <= r[1] < t1])
+    print("| kernel | launches (all) | mean us (all) | launches (timed steps) | mean us (timed steps) "
+          "| us per timed step |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for k in sorted(every, key=lambda k: -sum(window.get(k, every[k]))):
+        a, w = every[k], window.get(k, [])
+        print("| %s | %d | %.2f | %d | %s | %s |" % (
+            k, len(a), sum(a) / len(a), len(w), "%.2f" % (sum(w) / len(w)) if w else "-",
+            "%.1f" % (sum(w) / steps) if w else "-"))
+    if t0 is not None:
+        print("\ntimed window: %d steps, %.3f ms of trace time (%.3f ms per step)" % (
+            steps, (t1 - t0) / 1e6, (t1 - t0) / 1e6 / steps))
 
 
 def cpu_baseline(scene, mc, budget_s=20.0):
@@ -213,7 +285,8 @@ def bench_batched(n_scenes, steps, warmup, seed=3, log=None):
                           sc[0].iterations),
            "unit": "particle-steps/s", "steps": steps,
            "plain_per_scene_fp32": total * steps / plain_s}
-    for tag, flags in (("batched_graph_fp32", 0), ("batched_graph_mixed", MIXED_FLAGS)):
+    for tag, flags in (("batched_graph_fp32", 0), ("batched_graph_mixed", MIXED_FLAGS),
+                       ("batched_graph_mixed_packed16", MIXED_FLAGS | lf.LF_FLAG_PACKED16)):
         ctx, tune = _batched_ctx(lf, sc, flags, log, tag)
         k, sec = _time_graph(ctx, steps, warmup)
         ctx.close()
@@ -223,6 +296,8 @@ def bench_batched(n_scenes, steps, warmup, seed=3, log=None):
     out["speedup_batched_fp32"] = out["batched_graph_fp32"] / out["plain_per_scene_fp32"]
     out["speedup_batched_mixed_vs_plain_fp32"] = (out["batched_graph_mixed"]
                                                   / out["plain_per_scene_fp32"])
+    out["speedup_batched_mixed_packed16_vs_plain_fp32"] = (out["batched_graph_mixed_packed16"]
+                                                           / out["plain_per_scene_fp32"])
     return out
 
 
@@ -275,11 +350,20 @@ def main():
                     help="scenes of the batched comparison (0: skip it)")
     ap.add_argument("--no-combined", action="store_true",
                     help="skip the 32-scene ~8M combined case")
+    ap.add_argument("--skin", type=float, default=None,
+                    help="neighbour-list skin (fraction of h) of the 1M runs (default: the "
+                         "library's, 0.2)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the per-kernel HBM traffic table of two rocprofv3 --pmc runs")
+    ap.add_argument("--summarize-trace", metavar="RESULTS",
+                    help="print the per-kernel table of a rocprofv3 --kernel-trace run of this "
+                         "script (results .db or kernel_trace.csv) over the --steps/--warmup window")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.steps, args.warmup)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -299,6 +383,8 @@ def main():
           file=sys.stderr, flush=True)
 
     ctx = lf.Context(0)
+    if args.skin is not None:
+        ctx.exec_config(256, False, args.skin)
     ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
 
     def one_step():
@@ -391,53 +477,78 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded jittered lattice dam break)",
-        "config": {"workload": "%s: plain FP32 PBF, %d particles, h=%g, %d iterations, XSPH %g, "
-                               "marching cubes %dx%dx%d density field every step"
-                               % (scene.name, n, scene.h, scene.iterations, scene.xsph_c,
-                                  *mc.samples),
-                   "particles": n, "iterations": scene.iterations,
-                   "mc_samples": list(mc.samples), "triangles_per_step": ntri},
+        "config": {"workload": "%s: plain FP32 PBF dam break, %d particles (a 128x128x64 "
+                               "jittered lattice: the 1,048,576 count kept, a 128^3 lattice would "
+                               "hold 2,097,152), world spacing %g, world h %g (scale %g; solver "
+                               "h %g), %gx%gx%g box, %d iterations, XSPH %g, dt 1/120, marching "
+                               "cubes on the %dx%dx%d density field every step; steps %d-%d of "
+                               "the simulation timed"
+                               % (scene.name, n, scene.spacing, scene.h * scene.scale,
+                                  scene.scale, scene.h, *scene.max_bound, scene.iterations,
+                                  scene.xsph_c, *mc.samples, args.warmup,
+                                  args.warmup + args.steps),
+                   "particles": n, "iterations": scene.iterations, "scale": scene.scale,
+                   "world_h": scene.h * scene.scale, "world_spacing": scene.spacing,
+                   "box": list(scene.max_bound), "mc_samples": list(mc.samples),
+                   "triangles_per_step": ntri},
         "roofline": {"bound": "hbm", "kernel": dom_name,
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic_of(dom_name), "alg_bytes_per_launch": dom_bytes,
+                     "byte_model": "per-launch compulsory bytes of the path's measurement table "
+                                   "(lambda: 21 n + 4 G)",
                      "avg_launch_us": round(dom_avg_s * 1e6, 2),
+                     "window": "the %d timed steps (simulation steps %d-%d), HIP events on the "
+                               "library's stream" % (args.steps, args.warmup,
+                                                     args.warmup + args.steps),
                      "selection": "largest total time among the main-stream kernels (the step's "
                                   "critical path; the other streams run beside it)"},
         "kernels": kernels,
         "kernels_pass": "the timed steps run beforehand from the same start with every launch timed",
+        "grid_build": {
+            "kernels": [k for k in GRID_BUILD if k in stats],
+            "us_per_step": round(sum(stats[k][1] for k in GRID_BUILD if k in stats)
+                                 / args.steps * 1000.0, 2),
+            "alg_bytes": grid_build_bytes(n, G),
+            "alg_GBs": round(grid_build_bytes(n, G) / (sum(stats[k][1] for k in GRID_BUILD
+                                                            if k in stats) / args.steps / 1e3)
+                             / 1e9, 1) if any(k in stats for k in GRID_BUILD) else None},
         "device_ms_per_step": device_ms / args.steps,
         "main_stream_ms_per_step": sum(v[1] for k, v in stats.items()
                                        if KERNEL_STREAM.get(k, "main") == "main") / args.steps,
     }
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
-    # the same 1M workload with the mixed-precision options
-    pm = scene.params(flags=MIXED_FLAGS)
-    ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
-    ctx.profile(False)
-    for _ in range(args.warmup):
-        ctx.step(pm, 1)
-        ctx.mc_sample(pm, mc)
-        ctx.mc_classify_count(mc)
-        ctx.mc_emit(pm, mc)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.step(pm, 1)
-        ctx.mc_sample(pm, mc)
-        ctx.mc_classify_count(mc)
-        ctx.mc_emit(pm, mc)
-    ctx.sync()
-    mixed_s = time.perf_counter() - t0
-    out["mixed_1m"] = {"flags": "FAST_MATH|KAHAN", "value": n * args.steps / mixed_s,
-                       "ms_per_step": 1000.0 * mixed_s / args.steps}
-    log("mixed 1M: %.3f ms/step" % (1000.0 * mixed_s / args.steps))
+    # the same 1M workload with the mixed-precision options (and with the
+    # 16-bit cell-relative neighbour positions)
+    for key, flags, tag in (("mixed_1m", MIXED_FLAGS, "FAST_MATH|KAHAN"),
+                            ("mixed_packed16_1m", MIXED_FLAGS | lf.LF_FLAG_PACKED16,
+                             "FAST_MATH|KAHAN|PACKED16")):
+        pm = scene.params(flags=flags)
+        ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
+        ctx.profile(False)
+        for _ in range(args.warmup):
+            ctx.step(pm, 1)
+            ctx.mc_sample(pm, mc)
+            ctx.mc_classify_count(mc)
+            ctx.mc_emit(pm, mc)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.step(pm, 1)
+            ctx.mc_sample(pm, mc)
+            ctx.mc_classify_count(mc)
+            ctx.mc_emit(pm, mc)
+        ctx.sync()
+        mixed_s = time.perf_counter() - t0
+        out[key] = {"flags": tag, "value": n * args.steps / mixed_s,
+                    "ms_per_step": 1000.0 * mixed_s / args.steps}
+        log("%s: %.3f ms/step" % (key, 1000.0 * mixed_s / args.steps))
     if args.batch_scenes > 0:
-        out["batched"] = bench_batched(args.batch_scenes, max(4, args.steps // 5), args.warmup,
+        out["batched"] = bench_batched(args.batch_scenes, max(20, args.steps), args.warmup,
                                        log=log)
     if not args.no_combined:
-        out["combined"] = bench_combined(max(4, args.steps // 10), 1, log)
+        out["combined"] = bench_combined(max(20, args.steps // 5), 1, log)
     copy_bytes = 1 << 28
     copy_ms = ctx.copy_probe(copy_bytes, reps=20)
     out["copy_probe"] = {"bytes_moved": 2 * copy_bytes, "ms": copy_ms,

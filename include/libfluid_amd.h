@@ -39,6 +39,10 @@ extern "C" {
 #define LF_ERR_CAPACITY (-4)   /* marching-cubes output exceeded its capacity */
 #define LF_ERR_STATE (-5)      /* call out of order (e.g. lambda before grid) */
 #define LF_ERR_NODEVICE (-6)   /* no HIP device */
+#define LF_ERR_DIVERGED (-7)   /* the state diverged: a particle's p* is not
+                                  finite, or a cell of the grid holds more
+                                  particles than the guard allows (lf_set_guard);
+                                  the counts are in lf_last_error and lf_health */
 
 /* ---- particle types (include/fluid/fluid.hpp:18-20) --------------------- */
 #define LF_FLUID 0
@@ -54,6 +58,13 @@ extern "C" {
                                   correctly rounded ones: results are no longer
                                   bit-identical to the CPU restatement, only
                                   close (mixed precision) */
+#define LF_FLAG_PACKED16 0x8u  /* the lambda and delta passes gather neighbour
+                                  positions as cell-relative 16-bit fixed-point
+                                  offsets (unit h/65536, with the low 5 bits of
+                                  the cell coordinate: 8 bytes instead of 16);
+                                  relative positions are exact to one unit per
+                                  axis: close to the FP32 path, not identical
+                                  (mixed precision) */
 
 /*
  * Parameters of one PBF step. Replaces ClSphConfig (oclsph_type.h:6-13), the
@@ -342,17 +353,35 @@ int lf_diagnostics(lf_ctx* ctx, uint32_t scene, lf_diag* out);
  * reuse off: every pass after the first enumerates cells). None of these
  * change results, only speed. */
 int lf_exec_config(lf_ctx* ctx, uint32_t threads, int persistent, float skin);
+/* The full execution configuration. particles_per_thread (1, 2 or 4): a
+ * workgroup of a launch without the work queue takes that many consecutive
+ * chunks, each thread one particle of each. mc_tile: the density field's
+ * sample tile per workgroup, 0 = 8x8x4 (x, y, z), 1 = 4x8x8, 2 = 8x4x8. None
+ * of these change results. */
+typedef struct lf_exec_params {
+  uint32_t threads;
+  int32_t persistent;
+  float skin;
+  uint32_t particles_per_thread;
+  uint32_t mc_tile;
+} lf_exec_params;
+int lf_exec_configure(lf_ctx* ctx, const lf_exec_params* e);
 typedef struct lf_tune_result {
   uint32_t threads;
   int32_t persistent;
   float skin;
   float ms_per_step;   /* best configuration's mean step time */
   uint32_t trials;     /* configurations timed */
+  uint32_t particles_per_thread;
+  uint32_t mc_tile;
+  float mc_ms;         /* best tile's density-field time (0: none sampled yet) */
 } lf_tune_result;
 /* Times `steps` steps (plus one warm-up) of the current state under each
- * configuration of threads x persistent x skin and keeps the fastest. The
- * state is restored after every trial: the call does not advance the
- * simulation. */
+ * configuration of threads x particles per thread (without the queue) or
+ * persistent (with it) x skin, keeps the fastest, then, when the context has
+ * sampled a density field before, times that field under each tile shape and
+ * keeps the fastest. The state is restored after every trial: the call does
+ * not advance the simulation. Refused while sources or drains are set. */
 int lf_autotune(lf_ctx* ctx, uint32_t steps, lf_tune_result* out);
 
 /* ---- HIP graphs --------------------------------------------------------------
@@ -395,6 +424,25 @@ int lf_grid_info(const lf_ctx* ctx, uint32_t* table_size, uint32_t extent[3],
 /* MC lattice geometry of the last lf_mc_sample. */
 int lf_mc_info(const lf_ctx* ctx, uint32_t samples[3], float origin[3],
                float step[3]);
+
+/* ---- divergence guard -----------------------------------------------------
+ * No reference counterpart: a diverged libfluid scene keeps stepping, ever
+ * slower (a cell holding every particle makes each pass quadratic). Each
+ * step's key pass counts the particles whose predicted p* is not finite and
+ * the grid build counts the cells of the table holding more than
+ * max_cell_particles particles (default 256; 0 = no limit). Once either count
+ * is non-zero, the pair passes (list build, lambda, delta, finalise, XSPH
+ * copies velocities through, diffusion) and the density field of later
+ * launches do nothing, so the state stays as it was at the start of the step
+ * in which it diverged (in that step's sorted order), and lf_pbf_step,
+ * lf_batch_step and lf_graph_launch return LF_ERR_DIVERGED (with the counts in
+ * lf_last_error) once the copy of the counts made after an earlier step call
+ * has reached the host (lf_sync waits for it). The counts accumulate until the
+ * next upload or lf_set_guard call. */
+int lf_set_guard(lf_ctx* ctx, uint32_t max_cell_particles);
+/* Synchronises; out[0] = particles with a non-finite p*, out[1] = cells above
+ * the guard, out[2] = the largest such cell's particle count. */
+int lf_health(lf_ctx* ctx, uint32_t out[3]);
 
 /* ---- timing ----------------------------------------------------------------
  * With profiling on, every kernel launch of the context is bracketed by HIP

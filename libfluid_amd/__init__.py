@@ -27,14 +27,14 @@ from . import _abi
 from ._abi import (LF_ARR_CASES, LF_ARR_DENSITY, LF_ARR_DIFFUSED, LF_ARR_FIELD, LF_ARR_KEYS,
                    LF_ARR_LAMBDA, LF_ARR_MCCOLOUR, LF_ARR_MCNORMAL, LF_ARR_ORDER, LF_ARR_POSITION,
                    LF_ARR_PSTAR, LF_ARR_TABLE, LF_ARR_TYPE, LF_ARR_VELOCITY, LF_FLAG_DIFFUSE,
-                   LF_FLAG_FAST_MATH, LF_FLAG_KAHAN, LF_FLUID, LF_MC_FIELD_DENSITY, LF_MC_FIELD_REFERENCE,
+                   LF_FLAG_FAST_MATH, LF_FLAG_KAHAN, LF_FLAG_PACKED16, LF_FLUID, LF_MC_FIELD_DENSITY, LF_MC_FIELD_REFERENCE,
                    LF_OBSTACLE, McParams, PbfParams, particle_dtype, triangle_dtype)
 
 __all__ = [
     "Context", "SphSolver", "Config", "Source", "Drain", "Query", "QueryResult", "WellSpec",
     "PbfParams", "McParams", "LfError", "pbf_params", "mc_reference_params",
     "mc_density_params", "particle_dtype", "triangle_dtype", "LF_FLUID", "LF_OBSTACLE",
-    "LF_FLAG_DIFFUSE", "LF_FLAG_KAHAN", "LF_FLAG_FAST_MATH",
+    "LF_FLAG_DIFFUSE", "LF_FLAG_KAHAN", "LF_FLAG_FAST_MATH", "LF_FLAG_PACKED16",
 ]
 
 
@@ -251,6 +251,17 @@ class Context:
     def sync(self):
         self._check(self._lib.lf_sync(self._h))
 
+    def set_guard(self, max_cell_particles):
+        """Divergence guard: cells above this many particles (0: no limit) stop
+        the pair passes; clears the health counters."""
+        self._check(self._lib.lf_set_guard(self._h, int(max_cell_particles)))
+
+    def health(self):
+        """(non-finite p* count, overfull cells, largest overfull cell)."""
+        out = (ctypes.c_uint32 * 3)()
+        self._check(self._lib.lf_health(self._h, out))
+        return tuple(int(v) for v in out)
+
     # -- batches, execution configuration, graphs ---------------------------
     def batch_upload(self, params, scenes):
         """Several scenes in one context. params: one PbfParams per scene;
@@ -298,14 +309,17 @@ class Context:
                     density_error_max=d.density_error_max, passes=d.passes,
                     pass_error=list(d.pass_error)[:max(0, d.passes)])
 
-    def exec_config(self, threads=256, persistent=False, skin=0.2):
-        self._check(self._lib.lf_exec_config(self._h, threads, 1 if persistent else 0, skin))
+    def exec_config(self, threads=256, persistent=False, skin=0.2, particles_per_thread=1,
+                    mc_tile=0):
+        e = _abi.ExecParams(threads, 1 if persistent else 0, skin, particles_per_thread, mc_tile)
+        self._check(self._lib.lf_exec_configure(self._h, ctypes.byref(e)))
 
     def autotune(self, steps=3):
         r = _abi.TuneResult()
         self._check(self._lib.lf_autotune(self._h, steps, ctypes.byref(r)))
         return dict(threads=r.threads, persistent=bool(r.persistent), skin=r.skin,
-                    ms_per_step=r.ms_per_step, trials=r.trials)
+                    particles_per_thread=r.particles_per_thread, mc_tile=r.mc_tile,
+                    ms_per_step=r.ms_per_step, mc_ms=r.mc_ms, trials=r.trials)
 
     def graph_capture(self, nsteps=2, m=None):
         self._check(self._lib.lf_graph_capture(self._h, nsteps,

@@ -17,6 +17,7 @@ LF_ERR_MORTON = -3
 LF_ERR_CAPACITY = -4
 LF_ERR_STATE = -5
 LF_ERR_NODEVICE = -6
+LF_ERR_DIVERGED = -7
 
 LF_FLUID = 0
 LF_OBSTACLE = 1
@@ -24,6 +25,7 @@ LF_OBSTACLE = 1
 LF_FLAG_DIFFUSE = 0x1
 LF_FLAG_KAHAN = 0x2
 LF_FLAG_FAST_MATH = 0x4
+LF_FLAG_PACKED16 = 0x8
 
 LF_MC_FIELD_REFERENCE = 0
 LF_MC_FIELD_DENSITY = 1
@@ -142,6 +144,20 @@ class TuneResult(ctypes.Structure):
         ("skin", ctypes.c_float),
         ("ms_per_step", ctypes.c_float),
         ("trials", ctypes.c_uint32),
+        ("particles_per_thread", ctypes.c_uint32),
+        ("mc_tile", ctypes.c_uint32),
+        ("mc_ms", ctypes.c_float),
+    ]
+
+
+class ExecParams(ctypes.Structure):
+    """lf_exec_params: the execution configuration."""
+    _fields_ = [
+        ("threads", ctypes.c_uint32),
+        ("persistent", ctypes.c_int32),
+        ("skin", ctypes.c_float),
+        ("particles_per_thread", ctypes.c_uint32),
+        ("mc_tile", ctypes.c_uint32),
     ]
 
 
@@ -178,6 +194,8 @@ SIGNATURES = {
     "lf_ctx_destroy": (_I, [_P]),
     "lf_last_error": (ctypes.c_char_p, [_P]),
     "lf_sync": (_I, [_P]),
+    "lf_set_guard": (_I, [_P, ctypes.c_uint32]),
+    "lf_health": (_I, [_P, ctypes.POINTER(ctypes.c_uint32)]),
     "lf_state_upload": (_I, [_P, _SZ, _P, _P, _P, _P, _P]),
     "lf_state_download": (_I, [_P, _SZ, _P, _P, _P, _P, _P, _P]),
     "lf_state_size": (_I, [_P, ctypes.POINTER(_SZ)]),
@@ -220,6 +238,7 @@ SIGNATURES = {
     "lf_batch_info": (_I, [_P, ctypes.POINTER(_U32), _P]),
     "lf_batch_select": (_I, [_P, _U32]),
     "lf_exec_config": (_I, [_P, _U32, _I, ctypes.c_float]),
+    "lf_exec_configure": (_I, [_P, ctypes.POINTER(ExecParams)]),
     "lf_autotune": (_I, [_P, _U32, ctypes.POINTER(TuneResult)]),
     "lf_graph_capture": (_I, [_P, _U32, _P]),
     "lf_graph_launch": (_I, [_P, _U32]),

@@ -65,6 +65,14 @@ int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string
   }
   c.flags = p.flags;
   c.densityTol = p.density_tol > 0.0f ? p.density_tol : 0.0f;
+  c.qUnit = c.h / 65536.0f;
+  c.qInv = 65536.0 / double(c.h);
+  if (p.flags & LF_FLAG_PACKED16) {
+    // relative positions carry up to one quantum of error per axis: the list
+    // reuse test keeps two quanta of margin (sqrt(3) would do)
+    c.skinSafe -= 2.0f * c.qUnit;
+    if (!(c.skinSafe > 0.0f)) c.skinSafe = 0.0f;
+  }
   c.wdq = hostW6(c, p.corr_dq * c.h);
   c.wdqRecip = 1.0f / c.wdq;
   // rmax2: the largest float x with sqrtf(x) <= h
@@ -148,10 +156,10 @@ int prepareScenes(Ctx* c) {
   if (!c->batch) {
     PbfConsts k = c->consts;
     k.keyBase = 0;
-    k.clampKeys = 0;
+    k.pad0 = 0;
     consts.push_back(k);
     begin = {0u, uint32_t(c->n)};
-    total = k.G;
+    total = uint64_t(k.G) + 1u;
   } else {
     const size_t S = c->sceneParams.size();
     consts.resize(S);
@@ -160,8 +168,8 @@ int prepareScenes(Ctx* c) {
       const int rc = foldConsts(c->sceneParams[s], c->skinFrac, consts[s], err);
       if (rc) return fail(c, rc, "scene " + std::to_string(s) + ": " + err);
       consts[s].keyBase = uint32_t(total);
-      consts[s].clampKeys = S > 1 ? 1u : 0u;
-      total += uint64_t(consts[s].G) + (S > 1 ? 1u : 0u);
+      consts[s].pad0 = 0;
+      total += uint64_t(consts[s].G) + 1u;
     }
     begin = c->sceneBegin;
     if (total >= (uint64_t(1) << 30))
@@ -172,7 +180,11 @@ int prepareScenes(Ctx* c) {
                                       consts.size() * sizeof(PbfConsts)) == 0;
   const bool sameChunks = begin == c->uploadedBegin && c->chunksBuiltFor == c->chunkThreads;
   c->sceneConsts = consts;
-  c->tableTotal = uint32_t(total);
+  // buckets of the sort: every scene's cells and its bucket of keys >= G; the
+  // table a caller reads (LF_ARR_TABLE) is a single scene's G cells, or every
+  // bucket of a batch
+  c->bucketTotal = uint32_t(total);
+  c->tableTotal = c->batch ? uint32_t(total) : consts[0].G;
   c->converge = false;
   for (const PbfConsts& k : consts) c->converge = c->converge || k.densityTol > 0.0f;
   if (!c->batch) c->sceneBegin = begin;
@@ -229,6 +241,8 @@ int prepareScenes(Ctx* c) {
 
 Work workOf(Ctx* c, int queueSlot, bool skipDone) {
   Work W{};
+  W.halt = nullptr;
+  W.ppt = c->chunkPpt;
   W.skip = (skipDone && c->converge) ? c->sceneDone.p : nullptr;
   W.consts = c->dConsts.p;
   W.chunks = c->chunks.p;
@@ -237,6 +251,29 @@ Work workOf(Ctx* c, int queueSlot, bool skipDone) {
                 ? c->queues.p + 2 * queueSlot
                 : nullptr;
   return W;
+}
+
+int checkHealth(Ctx* c, bool wait) {
+  if (!c->healthHost) return LF_OK;
+  if (wait) LF_HIP(c, hipStreamSynchronize(c->stream));
+  // the copy k_reorder writes: up to date after a wait, else at most the
+  // steps still in flight behind
+  const uint32_t bad = c->healthHost[0], full = c->healthHost[1];
+  if (bad == 0 && full == 0) return LF_OK;
+  return fail(c, LF_ERR_DIVERGED,
+              "the state diverged: " + std::to_string(bad) + " particle(s) with a non-finite p*, " +
+                  std::to_string(full) + " cell(s) above " + std::to_string(c->guardCell) +
+                  " particles (largest " + std::to_string(c->healthHost[2]) +
+                  "); the pair passes stop until a new state is uploaded");
+}
+
+// Clears the health counters (a new state), after the steps in flight (they
+// write the host copy).
+static int resetHealth(Ctx* c) {
+  if (c->health.p) LF_HIP(c, hipMemsetAsync(c->health.p, 0, 4 * sizeof(uint32_t), c->stream));
+  LF_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->healthHost) std::memset(c->healthHost, 0, 4 * sizeof(uint32_t));
+  return LF_OK;
 }
 
 static int requireGrid(Ctx* c, const char* who) {
@@ -387,9 +424,14 @@ int lf_ctx_destroy(lf_ctx* c) {
   }
   lf::release(c->P);
   lf::release(c->PL);
+  lf::release(c->Q);
+  lf::release(c->QL);
+  lf::release(c->LAM);
   lf::release(c->nbr);
   lf::release(c->nbrCount);
   lf::release(c->P0);
+  lf::release(c->listEpoch);
+  lf::release(c->listR);
   lf::release(c->maxDisp);
   lf::release(c->cellDisp);
   lf::release(c->Palt);
@@ -397,7 +439,11 @@ int lf_ctx_destroy(lf_ctx* c) {
   lf::release(c->rho);
   lf::release(c->diffused);
   lf::release(c->table);
-  lf::release(c->sortHist);
+  lf::release(c->cellCount);
+  lf::release(c->bucketTile);
+  lf::release(c->bigBuckets);
+  lf::release(c->health);
+  if (c->healthHost) (void)hipHostFree(c->healthHost);
   lf::release(c->wells);
   lf::release(c->colliders);
   lf::release(c->colliderBounds);
@@ -443,7 +489,27 @@ int lf_sync(lf_ctx* c) {
   LF_HIP(c, hipStreamSynchronize(c->stream));
   if (c->mcStream) LF_HIP(c, hipStreamSynchronize(c->mcStream));
   lf::syncStreams(c);  // clears the pending stream waits
+  return lf::checkHealth(c, true);
+}
+
+int lf_health(lf_ctx* c, uint32_t out[3]) {
+  if (!c || !out) return LF_ERR_ARG;
+  LF_HIP(c, hipSetDevice(c->device));
+  uint32_t h[4] = {0, 0, 0, 0};
+  if (c->health.p) {
+    LF_HIP(c, hipMemcpyAsync(h, c->health.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    LF_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  for (int i = 0; i < 3; ++i) out[i] = h[i];
   return LF_OK;
+}
+
+int lf_set_guard(lf_ctx* c, uint32_t max_cell_particles) {
+  if (!c) return LF_ERR_ARG;
+  LF_HIP(c, hipSetDevice(c->device));
+  lf::dropGraph(c);  // a captured graph holds the old limit
+  c->guardCell = max_cell_particles;
+  return lf::resetHealth(c);
 }
 
 int lf_state_upload(lf_ctx* c, size_t n, const float* pos, const float* vel, const float* mass,
@@ -493,7 +559,7 @@ int lf_state_upload(lf_ctx* c, size_t n, const float* pos, const float* vel, con
   c->stepDone = false;
   c->mcSampled = c->mcCounted = c->mcEmitted = false;
   lf::dropGraph(c);
-  return LF_OK;
+  return lf::resetHealth(c);
 }
 
 int lf_state_download(lf_ctx* c, size_t n, float* pos, float* vel, float* mass, int32_t* type,
@@ -655,6 +721,7 @@ int lf_pbf_step(lf_ctx* c, const lf_pbf_params* p, uint32_t nsteps) {
   if (rc) return rc;
   c->params = *p;
   LF_HIP(c, hipSetDevice(c->device));
+  if ((rc = lf::checkHealth(c, false))) return rc;
   for (uint32_t s = 0; s < nsteps; ++s)
     if ((rc = lf::pbfStepOnce(c))) return rc;
   return LF_OK;
@@ -773,10 +840,16 @@ int lf_mc_set_capacity(lf_ctx* c, size_t max_triangles) {
   if (!c || max_triangles == 0) return LF_ERR_ARG;
   if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   if (max_triangles != c->triCap) {
+    // nothing in flight may still write the old buffers, and a captured graph
+    // holds their addresses and the old capacity: it goes with them
+    LF_HIP(c, hipSetDevice(c->device));
+    lf::syncStreams(c);
+    lf::dropGraph(c);
     lf::release(c->triPos);
     lf::release(c->triNrm);
     lf::release(c->triCol);
     c->triCap = max_triangles;
+    c->mcEmitted = false;  // the triangles went with the buffers; the counts stay valid
   }
   return LF_OK;
 }
@@ -902,8 +975,15 @@ int lf_write(lf_ctx* c, int array, const void* src, size_t bytes) {
     LF_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     if ((array == LF_ARR_PSTAR || array == LF_ARR_LAMBDA) && c->maxDisp.p) {
       // positions changed outside the delta pass: no neighbour list is usable
-      static const uint32_t kInf = 0x7F800000u;
-      LF_HIP(c, hipMemcpyAsync(c->maxDisp.p, &kInf, 4, hipMemcpyHostToDevice, c->stream));
+      // (D and every pass's D_j infinite)
+      static const uint32_t kInf[20] = {0x7F800000u, 0, 0, 0, 0x7F800000u, 0x7F800000u,
+                                        0x7F800000u, 0x7F800000u, 0x7F800000u, 0x7F800000u,
+                                        0x7F800000u, 0x7F800000u, 0x7F800000u, 0x7F800000u,
+                                        0x7F800000u, 0x7F800000u, 0x7F800000u, 0x7F800000u,
+                                        0x7F800000u, 0x7F800000u};
+      LF_HIP(c, hipMemcpyAsync(c->maxDisp.p, &kInf[0], 4, hipMemcpyHostToDevice, c->stream));
+      LF_HIP(c, hipMemcpyAsync(c->maxDisp.p + 4, &kInf[4], 16 * 4, hipMemcpyHostToDevice,
+                               c->stream));
       c->listFresh = false;
     }
     LF_HIP(c, hipStreamSynchronize(c->stream));
@@ -1011,10 +1091,10 @@ int lf_batch_step(lf_ctx* c, uint32_t nsteps) {
   if (!c) return LF_ERR_ARG;
   if (!c->batch) return fail(c, LF_ERR_STATE, "lf_batch_step: no batch (call lf_batch_upload)");
   LF_HIP(c, hipSetDevice(c->device));
-  for (uint32_t s = 0; s < nsteps; ++s) {
-    const int rc = lf::pbfStepOnce(c);
-    if (rc) return rc;
-  }
+  int rc = lf::checkHealth(c, false);
+  if (rc) return rc;
+  for (uint32_t s = 0; s < nsteps; ++s)
+    if ((rc = lf::pbfStepOnce(c))) return rc;
   return LF_OK;
 }
 
@@ -1041,19 +1121,30 @@ int lf_batch_select(lf_ctx* c, uint32_t scene) {
   return LF_OK;
 }
 
-int lf_exec_config(lf_ctx* c, uint32_t threads, int persistent, float skin) {
-  if (!c) return LF_ERR_ARG;
-  if (threads != 128 && threads != 256 && threads != 512)
-    return fail(c, LF_ERR_ARG, "lf_exec_config: threads must be 128, 256 or 512");
-  if (!(skin >= 0.0f && skin <= 1.0f))
-    return fail(c, LF_ERR_ARG, "lf_exec_config: skin must be in [0, 1] (fraction of h)");
+int lf_exec_configure(lf_ctx* c, const lf_exec_params* e) {
+  if (!c || !e) return LF_ERR_ARG;
+  if (e->threads != 128 && e->threads != 256 && e->threads != 512)
+    return fail(c, LF_ERR_ARG, "lf_exec_configure: threads must be 128, 256 or 512");
+  if (!(e->skin >= 0.0f && e->skin <= 1.0f))
+    return fail(c, LF_ERR_ARG, "lf_exec_configure: skin must be in [0, 1] (fraction of h)");
+  if (e->particles_per_thread != 1 && e->particles_per_thread != 2 && e->particles_per_thread != 4)
+    return fail(c, LF_ERR_ARG, "lf_exec_configure: particles_per_thread must be 1, 2 or 4");
+  if (e->mc_tile > 2) return fail(c, LF_ERR_ARG, "lf_exec_configure: mc_tile must be 0, 1 or 2");
   lf::dropGraph(c);
-  c->chunkThreads = threads;
-  c->persistent = persistent != 0;
-  c->skinFrac = skin;
+  c->chunkThreads = e->threads;
+  c->persistent = e->persistent != 0;
+  c->skinFrac = e->skin;
+  c->chunkPpt = e->particles_per_thread;
+  c->mcTile = e->mc_tile;
   c->gridValid = false;  // constants fold the skin: the next call must step or build the grid
   if (!c->batch && c->params.h > 0.0f) (void)lf::setParams(c, &c->params);
   return LF_OK;
+}
+
+int lf_exec_config(lf_ctx* c, uint32_t threads, int persistent, float skin) {
+  if (!c) return LF_ERR_ARG;
+  const lf_exec_params e{threads, persistent, skin, c->chunkPpt, c->mcTile};
+  return lf_exec_configure(c, &e);
 }
 
 int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
@@ -1061,6 +1152,9 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
   c->posEventValid = false;  // P and the table are rewritten outside the event
   if (c->n == 0 || !(c->params.h > 0.0f))
     return fail(c, LF_ERR_STATE, "lf_autotune: needs a state and one step (for its parameters)");
+  // trial steps would append and erase particles: the state could not be put back
+  if (!c->sources.empty() || !c->hostDrains.empty())
+    return fail(c, LF_ERR_STATE, "lf_autotune: sources and drains change the particle count");
   LF_HIP(c, hipSetDevice(c->device));
   const size_t n = c->n;
   const int cur0 = c->cur;
@@ -1111,15 +1205,23 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
   }
   const bool prof = c->prof;
   c->prof = false;
+  // trial steps leave the last step's query answers alone
+  const uint32_t nQueries = c->nQueries;
+  c->nQueries = 0;
   const uint32_t threadsList[3] = {128, 256, 512};
   const float skins[3] = {0.1f, 0.2f, 0.3f};
+  // (particles per thread, persistent): the queue takes chunks one at a time
+  const uint32_t pptPers[4][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}};
   lf_tune_result best{};
   best.ms_per_step = INFINITY;
+  best.mc_tile = c->mcTile;
   uint32_t trials = 0;
+  const uint32_t tile0 = c->mcTile;
   for (uint32_t th : threadsList)
-    for (int pers = 0; pers < 2 && rc == LF_OK; ++pers)
+    for (int pp = 0; pp < 4 && rc == LF_OK; ++pp)
       for (float sk : skins) {
-        if ((rc = lf_exec_config(c, th, pers, sk))) break;
+        const lf_exec_params ex{th, int32_t(pptPers[pp][1]), sk, pptPers[pp][0], tile0};
+        if ((rc = lf_exec_configure(c, &ex))) break;
         if (!copyState(false)) break;
         if ((rc = lf::pbfStepOnce(c))) break;  // warm-up (allocations, caches)
         if (!hipOk(hipEventRecord(e0, c->stream), "autotune")) break;
@@ -1133,13 +1235,48 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
         ++trials;
         if (ms / float(steps) < best.ms_per_step) {
           best.threads = th;
-          best.persistent = pers;
+          best.persistent = int32_t(pptPers[pp][1]);
           best.skin = sk;
+          best.particles_per_thread = pptPers[pp][0];
           best.ms_per_step = ms / float(steps);
         }
       }
+  if (rc == LF_OK) {
+    const lf_exec_params ex{best.threads, best.persistent, best.skin, best.particles_per_thread,
+                            tile0};
+    rc = lf_exec_configure(c, &ex);
+  }
+  // the density field's tile shape, on the last density lattice sampled, from
+  // one step of the restored state
+  if (rc == LF_OK && c->haveDensity && copyState(false) && (rc = lf::pbfStepOnce(c)) == LF_OK) {
+    const lf_mc_params m = c->lastDensity;
+    float bestMc = INFINITY;
+    for (uint32_t t = 0; t < 3 && rc == LF_OK; ++t) {
+      c->mcTile = t;
+      if ((rc = lf::mcSample(c, m))) break;  // warm-up
+      if ((rc = lf::mcJoin(c))) break;
+      if (!hipOk(hipEventRecord(e0, c->stream), "autotune")) break;
+      for (int r = 0; r < 3 && rc == LF_OK; ++r) {
+        rc = lf::mcSample(c, m);
+        if (rc == LF_OK) rc = lf::mcJoin(c);
+      }
+      if (rc) break;
+      if (!hipOk(hipEventRecord(e1, c->stream), "autotune") ||
+          !hipOk(hipEventSynchronize(e1), "autotune"))
+        break;
+      float ms = 0.0f;
+      if (!hipOk(hipEventElapsedTime(&ms, e0, e1), "autotune")) break;
+      ++trials;
+      if (ms / 3.0f < bestMc) {
+        bestMc = ms / 3.0f;
+        best.mc_tile = t;
+        best.mc_ms = bestMc;
+      }
+    }
+    c->mcTile = best.mc_tile;
+  }
   c->prof = prof;
-  if (rc == LF_OK) rc = lf_exec_config(c, best.threads, best.persistent, best.skin);
+  c->nQueries = nQueries;
   if (rc == LF_OK) (void)copyState(false);
   if (rc == LF_OK) (void)hipOk(hipStreamSynchronize(c->stream), "autotune");
   cleanup();
@@ -1205,6 +1342,8 @@ int lf_graph_launch(lf_ctx* c, uint32_t times) {
   if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   if (!c->graph) return fail(c, LF_ERR_STATE, "lf_graph_launch: no captured graph");
   LF_HIP(c, hipSetDevice(c->device));
+  int rc = lf::checkHealth(c, false);
+  if (rc) return rc;
   for (uint32_t t = 0; t < times; ++t) LF_HIP(c, hipGraphLaunch(c->graph, c->stream));
   return LF_OK;
 }

@@ -52,13 +52,51 @@ struct PbfConsts {
   uint32_t G;        // dense cell-table size Morton(extent) (oclsph.hpp:749)
   uint32_t extent[3];
   uint32_t flags;
-  // batches: this scene's slice of the concatenated cell tables. The sort key
-  // is keyBase + key, with key clamped to G when clampKeys (keys >= G belong
-  // to no cell), so scenes sort apart and each keeps the reference's order.
+  // this scene's slice of the concatenated cell tables of a batch (0 for a
+  // single scene). The sort key is keyBase + min(key, G): keys >= G belong to
+  // no cell and share the scene's last bucket, so scenes sort apart and each
+  // keeps the reference's order of its cells.
   uint32_t keyBase;
-  uint32_t clampKeys;
+  uint32_t pad0;
   float densityTol;  // > 0: skip the remaining passes once the density error is below it
+  float qUnit;       // LF_FLAG_PACKED16: one fixed-point quantum, h / 65536 (solver units)
+  double qInv;       // 65536 / h
 };
+
+// ---- LF_FLAG_PACKED16: cell-relative 16-bit fixed-point positions ---------------
+// The lambda and delta passes gather their neighbours' p* as one 8-byte
+// record instead of a 16-byte float4: per axis the 21-bit fixed-point
+// coordinate round((p - minExt) * 65536 / h) mod 2^21, i.e. the 16-bit offset
+// of p within its cell (unit h / 65536) and the low 5 bits of its cell
+// coordinate. The difference of two such coordinates, taken mod 2^21 and
+// sign-extended, is p_a - p_b in quanta exactly (pairs closer than 16 cells on
+// every axis: candidates are within one cell at the list build), so each
+// relative position is the true one to one quantum per axis, a finer grain
+// than FP32 absolute coordinates far from the origin.
+constexpr uint32_t kQBits = 21;
+constexpr uint32_t kQMask = (1u << kQBits) - 1u;
+__device__ inline uint32_t quantAxis(const PbfConsts& c, float p, int k) {
+  const double q = rint((double(p) - double(c.minExt[k])) * c.qInv);
+  // only positions far outside the grid leave the int32 range: clamped
+  const double qc = fmin(fmax(q, -1073741824.0), 1073741823.0);
+  return uint32_t(int32_t(qc)) & kQMask;
+}
+__device__ inline uint2 packPos(const PbfConsts& c, float x, float y, float z) {
+  const uint32_t qx = quantAxis(c, x, 0), qy = quantAxis(c, y, 1), qz = quantAxis(c, z, 2);
+  return make_uint2(qx | (qy << kQBits), (qy >> (32 - kQBits)) | (qz << (2 * kQBits - 32)));
+}
+struct QPos {
+  uint32_t x, y, z;
+};
+__device__ inline QPos unpackPos(uint2 r) {
+  return QPos{r.x & kQMask, ((r.x >> kQBits) | (r.y << (32 - kQBits))) & kQMask,
+              (r.y >> (2 * kQBits - 32)) & kQMask};
+}
+// a - b of two fixed-point coordinates, in solver units
+__device__ inline float qDiff(uint32_t a, uint32_t b, float unit) {
+  const int32_t d = int32_t((a - b) << (32 - kQBits)) >> (32 - kQBits);
+  return float(d) * unit;
+}
 
 // ---- work decomposition -------------------------------------------------------
 // Chunks of at most the workgroup size of consecutive particles of one scene
@@ -73,6 +111,10 @@ struct Work {
   uint32_t nChunks;
   uint32_t* queue;  // persistent launch: [0] next chunk, [1] blocks done; null: a workgroup per chunk
   const uint32_t* skip;  // per scene: non-zero skips the scene's chunks (converged); may be null
+  // non-null: the health counters of the step (non-finite p*, overfull
+  // cells); the launch does nothing once either is raised (a diverged state)
+  const uint32_t* halt;
+  uint32_t ppt;  // chunks per workgroup of a launch without a queue (particles per thread)
 };
 
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD
@@ -90,13 +132,22 @@ __device__ inline uint32_t xcdChunk(uint32_t b, uint32_t n) {
 // leave resets the counters for the next launch, so no host-side clear is
 // needed (graph-safe).
 template <typename Body> __device__ inline void forEachChunk(const Work& W, Body&& body) {
+  if (W.halt && (W.halt[0] | W.halt[1])) return;  // uniform: before any chunk is taken
   // The constants are copied out of global memory once per chunk: a reference
   // into global memory would be re-read after every store the body makes.
   if (!W.queue) {
-    const Chunk ch = W.chunks[xcdChunk(blockIdx.x, W.nChunks)];
-    if (W.skip && W.skip[ch.scene]) return;
-    const PbfConsts c = W.consts[ch.scene];
-    body(c, ch);
+    // workgroup b takes chunks ppt*x .. ppt*x + ppt - 1, x = xcdChunk(b): one
+    // after another (each thread its particle of each), so a thread handles
+    // up to ppt particles; the trip count is uniform over the workgroup
+    const uint32_t nGroups = (W.nChunks + W.ppt - 1u) / W.ppt;
+    const uint32_t first = xcdChunk(blockIdx.x, nGroups) * W.ppt;
+    for (uint32_t k = 0; k < W.ppt; ++k) {
+      if (first + k >= W.nChunks) break;
+      const Chunk ch = W.chunks[first + k];
+      if (W.skip && W.skip[ch.scene]) continue;
+      const PbfConsts c = W.consts[ch.scene];
+      body(c, ch);
+    }
     return;
   }
   __shared__ uint32_t next;
@@ -478,10 +529,18 @@ struct Ctx {
   DevBuf<uint32_t> ID[2];     // upload index
   DevBuf<uint32_t> sortKey[2], sortVal[2];
   DevBuf<float4> P, PL;       // p* (+mass), p* + lambda
+  // LF_FLAG_PACKED16: p* as 8-byte fixed-point records (Q: current, QL: the
+  // copy the lambda pass makes for the delta pass) and lambda alone
+  DevBuf<uint2> Q, QL;
+  DevBuf<float> LAM;
+  bool gridPacked = false;    // the last grid build wrote Q
   DevBuf<float> rho;          // density of the last lambda pass
   DevBuf<uint32_t> nbr;       // neighbour lists (nbrGroupByte layout)
   DevBuf<uint32_t> nbrCount;  // list length per particle (> kMaxNbr: overflowed)
   DevBuf<float4> P0;          // p* at the list build
+  DevBuf<uint8_t> listEpoch;  // per particle: the pass its list was (re)built from
+  DevBuf<float4> listR;       // per particle: p* of that pass (epoch > 1)
+  int passIdx = 1;            // lambda/delta pass of the step (1-based), XSPH: iterations + 1
   DevBuf<uint32_t> maxDisp;   // [0]: float bits of the largest displacement since
   DevBuf<uint32_t> cellDisp;  // per cell: float bits of its particles' largest one
   DevBuf<uint32_t> diffused;
@@ -495,7 +554,17 @@ struct Ctx {
     hipEvent_t ev = nullptr;
     bool pending = false;
   } pairGuard, pairGuardAlt;
-  DevBuf<uint32_t> sortHist;
+  // grid build (lf_grid.hip): bucket counts (all zero between steps), scan
+  // tile sums, the list of buckets too large for the in-place ordering
+  DevBuf<uint32_t> cellCount, bucketTile, bigBuckets;
+  // health of the state: [0] particles whose p* is not finite, [1] cells of
+  // the table holding more than guardCell particles, [2] the largest such
+  // count; accumulated since the last upload. Once [0] or [1] is raised the
+  // pair passes and the density field do nothing and the step calls report
+  // LF_ERR_DIVERGED. healthHost: a pinned host copy each grid build writes.
+  DevBuf<uint32_t> health;
+  uint32_t guardCell = 256;
+  uint32_t* healthHost = nullptr;
   DevBuf<lf_well> wells;
   int nWells = 0;
   DevBuf<ColliderTri> colliders;
@@ -531,11 +600,16 @@ struct Ctx {
   std::vector<lf_pbf_params> sceneParams;  // per scene
   std::vector<PbfConsts> sceneConsts;      // per scene, folded
   std::vector<uint32_t> sceneBegin;        // particle offsets, nScenes + 1 entries
-  uint32_t tableTotal = 0;                 // concatenated table size
+  uint32_t tableTotal = 0;                 // concatenated table size (LF_ARR_TABLE)
+  uint32_t bucketTotal = 0;                // sort buckets: every scene's G cells + 1
   DevBuf<PbfConsts> dConsts;
   DevBuf<Chunk> chunks;
   uint32_t nChunks = 0;
   uint32_t chunkThreads = 256;             // workgroup size of the particle kernels
+  uint32_t chunkPpt = 1;                   // chunks per workgroup (particles per thread)
+  uint32_t mcTile = 0;                     // density-field tile shape (lf_exec_params)
+  lf_mc_params lastDensity{};              // the last density-field lattice sampled (autotuner)
+  bool haveDensity = false;
   std::vector<PbfConsts> uploadedConsts;  // what dConsts holds
   std::vector<uint32_t> uploadedBegin;     // scene offsets of the uploaded chunk list
   uint32_t chunksBuiltFor = 0;             // chunkThreads the chunk list was built with
@@ -556,7 +630,7 @@ struct Ctx {
   bool capturing = false;
   hipGraphExec_t graph = nullptr;
   int graphSteps = 0;
-  int keysIdx = 0;  // which sortKey/sortVal holds the sorted result
+  int keysIdx = 0;  // which sortKey/sortVal holds the sorted keys and order
 
   // marching cubes
   DevBuf<float> field;
@@ -603,6 +677,22 @@ inline void syncStreams(Ctx* c) {
   c->mcBusy = false;  // every event recorded so far has completed
   c->pairGuard.pending = c->pairGuardAlt.pending = false;
 }
+
+// Before a kernel that writes the current P in place on the main stream (the
+// delta and finalise passes): a density field sampled from it since the grid
+// build may still be reading it on the marching-cubes stream.
+inline int waitPairReaders(Ctx* c) {
+  if (!c->pairGuard.pending) return 0;
+  c->pairGuard.pending = false;
+  if (hipStreamWaitEvent(c->stream, c->pairGuard.ev, 0) != hipSuccess)
+    return fail(c, LF_ERR_HIP, "waiting for the density field's read of p*");
+  return 0;
+}
+
+// Health of the state (see Ctx::health): reports a raised counter of the
+// pinned host copy as LF_ERR_DIVERGED; without `wait` the copy may lag the
+// steps still in flight, with it the stream is synchronised first.
+int checkHealth(Ctx* c, bool wait);
 
 // After a kernel that writes P or the table on the main stream.
 inline void notePositionsWritten(Ctx* c) {
@@ -691,7 +781,7 @@ inline unsigned blocksFor(size_t n, unsigned per) { return unsigned((n + per - 1
 // grid of at most persistentBlocks workgroups when the launch takes its chunks
 // from a work queue.
 inline dim3 gridOf(const Ctx* c, const Work& W) {
-  if (!W.queue) return dim3(W.nChunks);
+  if (!W.queue) return dim3((W.nChunks + W.ppt - 1u) / W.ppt);
   return dim3(W.nChunks < c->persistentBlocks ? W.nChunks : c->persistentBlocks);
 }
 
@@ -716,7 +806,6 @@ inline dim3 gridOf(const Ctx* c, const Work& W) {
 int prepareScenes(Ctx* c);                    // lf_api.hip: consts + chunks to the device
 Work workOf(Ctx* c, int queueSlot, bool skipDone = false);  // queueSlot < 0: one workgroup per chunk
 int gridBuild(Ctx* c);                        // lf_grid.hip
-int radixSort(Ctx* c, size_t n, int& outIdx);  // lf_sort.hip
 int pbfBuildLists(Ctx* c);                    // lf_pbf.hip
 int pbfLambda(Ctx* c, bool skipDone);  // skipDone: skip the scenes that converged
 int pbfDelta(Ctx* c, bool skipDone, bool final = false);  // final: finalise fused in

@@ -56,18 +56,13 @@ struct Lattice {
 // a local cell table, so the per-sample candidate loop reads LDS; a tile whose
 // halo holds no particle writes zeros and exits. Tiles whose halo does not fit
 // (coarse lattices, crowded cells) read global memory instead, same order.
-// 8 x 8 x 4 samples (x, y, z) per tile: measured fastest of 12 shapes at
-// 256^3 on the 1M dam break (373-391 us against 597-764 us for 4 x 8 x 8)
-#ifndef LF_MC_TILE_X
-#define LF_MC_TILE_X 8
-#define LF_MC_TILE_Y 8
-#define LF_MC_TILE_Z 4
-#endif
+// The tile shape is a runtime choice (lf_exec_configure, the autotuner):
+// 8 x 8 x 4 samples (x, y, z) by default, measured fastest of 12 shapes at
+// 256^3 on the 1M dam break (373-391 us against 597-764 us for 4 x 8 x 8);
+// 4 x 8 x 8 and 8 x 4 x 8 are the alternatives.
 #ifndef LF_MC_UNROLL
 #define LF_MC_UNROLL 4
 #endif
-constexpr int kTX = LF_MC_TILE_X, kTY = LF_MC_TILE_Y, kTZ = LF_MC_TILE_Z;  // 256 samples
-static_assert(kTX * kTY * kTZ == kThreads, "one sample per thread");
 constexpr uint32_t kUnroll = LF_MC_UNROLL;  // staged particles in flight per sample
 constexpr int kHaloCells = 512;
 constexpr int kStageCap = 2048;           // staged particles (32 KB)
@@ -96,11 +91,15 @@ __device__ inline void cellCoords(const PbfConsts& c, float px, float py, float 
   z = cellCoord((pz - c.minExt[2]) / c.h);
 }
 
+template <int kTX, int kTY, int kTZ>
 __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
                                                          const uint32_t* __restrict__ table,
                                                          const float4* __restrict__ P,
                                                          float* __restrict__ field,
-                                                         uint32_t tilesY, uint32_t tilesZ) {
+                                                         uint32_t tilesY, uint32_t tilesZ,
+                                                         const uint32_t* __restrict__ health) {
+  static_assert(kTX * kTY * kTZ == kThreads, "one sample per thread");
+  if (health[0] | health[1]) return;  // a diverged state: no field
   // staged particles as 16-byte (x, y, z, m) records: one LDS read per
   // candidate at a constant offset from one address (separate x/y/z/m arrays
   // 8 KB apart need extra address arithmetic)
@@ -286,9 +285,10 @@ __global__ __launch_bounds__(kThreads) void k_mc_reference(
     uint64_t total, Lattice L, PbfConsts c, float res, float psize, float infl,
     const uint32_t* __restrict__ table, const uint8_t* __restrict__ T,
     const float4* __restrict__ X, const float4* __restrict__ V, float* __restrict__ field,
-    float4* __restrict__ normalOut, uint32_t* __restrict__ colourOut) {
+    float4* __restrict__ normalOut, uint32_t* __restrict__ colourOut,
+    const uint32_t* __restrict__ health) {
   const uint64_t idx = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (idx >= total) return;
+  if (idx >= total || (health[0] | health[1])) return;  // a diverged state: no field
   const uint32_t z = uint32_t(idx % L.S[2]);
   const uint64_t xy = idx / L.S[2];
   const uint32_t y = uint32_t(xy % L.S[1]);
@@ -738,10 +738,14 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
     rc = launch(c, "mc_reference_field", k_mc_reference, dim3(nb), dim3(kThreads), 0, total, L, k,
                 m.resolution, m.particle_size, m.particle_influence, table,
                 (const uint8_t*)c->T[c->cur].p, (const float4*)c->X[c->cur].p,
-                (const float4*)c->V[c->cur].p, c->field.p, c->mcNormal.p, c->mcColour.p);
+                (const float4*)c->V[c->cur].p, c->field.p, c->mcNormal.p, c->mcColour.p,
+                (const uint32_t*)c->health.p);
     c->fieldOnMain = true;
     c->mcAttr = 1;
   } else {
+    static const int kShapes[3][3] = {{8, 8, 4}, {4, 8, 8}, {8, 4, 8}};
+    const int* sh = kShapes[c->mcTile < 3 ? c->mcTile : 0];
+    const uint32_t kTX = uint32_t(sh[0]), kTY = uint32_t(sh[1]), kTZ = uint32_t(sh[2]);
     const uint32_t tX = (L.S[0] + kTX - 1) / kTX, tY = (L.S[1] + kTY - 1) / kTY,
                    tZ = (L.S[2] + kTZ - 1) / kTZ;
     const uint64_t tiles = uint64_t(tX) * tY * tZ;
@@ -749,8 +753,12 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
     // on the marching-cubes stream, after the step: it reads only P and the
     // table, which the next step leaves alone (it writes the other pair)
     OnMcStream onMc(c, McInput::Positions);
-    rc = launch(c, "mc_density_field", k_mc_density, dim3(unsigned(tiles)), dim3(kThreads), 0, L,
-                k, table, (const float4*)c->P.p, c->field.p, tY, tZ);
+    auto kern = c->mcTile == 1 ? k_mc_density<4, 8, 8>
+                : c->mcTile == 2 ? k_mc_density<8, 4, 8> : k_mc_density<8, 8, 4>;
+    rc = launch(c, "mc_density_field", kern, dim3(unsigned(tiles)), dim3(kThreads), 0, L, k, table,
+                (const float4*)c->P.p, c->field.p, tY, tZ, (const uint32_t*)c->health.p);
+    c->lastDensity = m;
+    c->haveDensity = true;
     if (rc == LF_OK && onMc.on) {
       if (hipEventRecord(c->pairGuard.ev, c->mcStream) == hipSuccess)
         c->pairGuard.pending = true;

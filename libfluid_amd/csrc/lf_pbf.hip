@@ -60,28 +60,49 @@ struct Lists {
   uint32_t* count;         // entries per particle (> kMaxNbr: overflowed)
   float4* P0;              // clamped p* at the build
   uint32_t* maxDisp;       // [0]: float bits of D, the largest |p* - P0| so far;
-                           // [1..3]: fallbacks of the lambda, delta, XSPH passes
+                           // [1..3]: fallbacks of the lambda, delta, XSPH passes;
+                           // [4 + j]: float bits of D_j, the largest move of any
+                           // particle in delta pass j (1-based)
   uint32_t* cellDisp;      // per cell (G entries): float bits of the largest |p* - P0|
                            // of the particles of that cell
+  uint8_t* epoch;          // per particle: the pass whose positions its list was
+                           // built from (1: the step's build from P0)
+  float4* R;               // per particle: its p* at that pass (epoch > 1)
 };
+constexpr int kPassSlots = 16;  // D_j kept for passes 1..kPassSlots - 1
 
 // Whether particle a may walk its list at position (x, y, z); `fresh` marks
 // the first lambda/delta pair after the build, where the lists are complete.
+// A list rebuilt from the positions of pass e (lambda pass e rebuilds the
+// lists of a wave that would otherwise walk its cells) holds every candidate
+// within h + skin at those positions; by pass p every particle has moved at
+// most sum_{j=e}^{p-1} D_j since, so the list stays complete while
+// |p*_a - R_a| + that sum < skin.
 __device__ inline bool listUsable(const PbfConsts& c, const Lists& L, uint32_t a, float x, float y,
-                                  float z, bool fresh, uint32_t& cnt) {
+                                  float z, bool fresh, int passIdx, uint32_t& cnt, uint32_t& ep) {
   cnt = L.count[a];
+  ep = L.epoch[a];
   if (cnt > kMaxNbr) return false;
   if (fresh) return true;
+  if (ep > 1u) {
+    const float4 r = L.R[a];
+    float s = __builtin_sqrtf(dist2(x - r.x, y - r.y, z - r.z));
+    for (int j = int(ep); j < passIdx && j < kPassSlots; ++j) s += __uint_as_float(L.maxDisp[4 + j]);
+    return passIdx < kPassSlots && s < c.skinSafe;
+  }
   const float4 p0 = L.P0[a];
   const float da = __builtin_sqrtf(dist2(x - p0.x, y - p0.y, z - p0.z));
   return da + __uint_as_float(L.maxDisp[0]) < c.skinSafe;
 }
 
 // The local form of the test: D replaced by the largest displacement among the
-// particles of the 27 cells around `key` (the only possible candidates).
+// particles of the 27 cells around `key` (the only possible candidates). A
+// list rebuilt at pass e > 1 (reference R_a) takes twice that: a candidate b
+// has moved at most |p*_b - P0_b| + |p*_b(e) - P0_b| since pass e, and the
+// per-cell value is the largest |p* - P0| of the step so far.
 __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint32_t a, float x,
-                                       float y, float z, uint32_t key) {
-  const float4 p0 = L.P0[a];
+                                       float y, float z, uint32_t key, uint32_t ep) {
+  const float4 p0 = ep > 1u ? L.R[a] : L.P0[a];
   const float da = __builtin_sqrtf(dist2(x - p0.x, y - p0.y, z - p0.z));
   const uint32_t xs[3] = {dilatedDec(key, kMortonX), key & kMortonX, dilatedInc(key, kMortonX)};
   const uint32_t ys[3] = {dilatedDec(key, kMortonY), key & kMortonY, dilatedInc(key, kMortonY)};
@@ -100,7 +121,8 @@ __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint3
     const uint32_t code = xs[i % 3] | ys[(i / 3) % 3] | zs[i / 9];
     m = max(m, code + 1u < c.G ? d[i] : 0u);
   }
-  return da + __uint_as_float(m) < c.skinSafe;
+  const float mb = __uint_as_float(m);
+  return da + (ep > 1u ? mb + mb : mb) < c.skinSafe;
 }
 
 // base[i] addressed as base + a 32-bit byte offset (the base stays in scalar
@@ -111,88 +133,121 @@ template <typename T> __device__ inline T at(const T* __restrict__ base, uint32_
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(T)));
 }
 
-// Neighbour lists of the predicted p* (see the file comment). P0 (the
-// clamped p*) was written by the reorder kernel. Each lane collects its
-// accepted candidates in a 16-entry LDS ring (two 8-entry groups) and, once
-// per block of candidates, stores a completed group as 24 bytes of its list:
-// a block adds at most 8 entries, so at most one group completes in it and the
-// group still to be stored is never overwritten. (Checking after every
-// candidate ran the store path on almost every step: with 64 lanes, some lane
-// nearly always completes a group.)
+// Gathered neighbour records: a float4 whose xyz is p*, position and velocity
+// together (XSPH), or p* and its fixed-point record (LF_FLAG_PACKED16 lists
+// rebuilt by the lambda pass).
+struct PosVel {
+  float4 p, v;
+};
+struct PosQ {
+  float4 p;
+  uint2 q;
+};
+__device__ inline const float4& posOf(const float4& r) { return r; }
+__device__ inline const float4& posOf(const PosVel& r) { return r.p; }
+__device__ inline const float4& posOf(const PosQ& r) { return r.p; }
+
+// Builds particle a's list in the reference's visit order (the 27 cells of
+// key, in sorted order within a cell): the candidates whose position posOf(
+// load(b)) lies within h + skin of pa. visit(b, record, valid) runs for each
+// candidate in that order (valid false for the padding of a block), so a pass
+// that rebuilds lists evaluates its pair terms in the same walk. Each lane
+// collects its accepted candidates in a 16-entry LDS ring (win, two 8-entry
+// groups) and, once per block of candidates, stores a completed group as 24
+// bytes of its list: a block adds at most kBuildBlock <= 8 entries, so at most
+// one group completes in it and the group still to be stored is never
+// overwritten. (Checking after every candidate ran the store path on almost
+// every step: with 64 lanes, some lane nearly always completes a group.) The
+// list is padded with a itself to whole blocks of kListPad entries; the
+// count (kMaxNbr + 1: overflowed, or a candidate 2^23 positions away) goes
+// to L.count[a].
+template <int BS, typename Load, typename Visit>
+__device__ inline void buildList(const PbfConsts& c, const Lists& L, uint32_t* __restrict__ win,
+                                 uint32_t a, float4 pa, uint32_t key,
+                                 const uint32_t* __restrict__ table, int checkFar, Load&& load,
+                                 Visit&& visit) {
+  constexpr uint32_t kRing = 2 * kNbrGroup;
+  // at most 7 entries wait before a block, so at most 15 after it: one group
+  // completes at most, and a block's writes always find a free slot
+  static_assert(kBuildBlock <= int(kNbrGroup), "a block completes at most one group");
+  uint32_t cnt = 0, stored = 0;  // entries accepted / written to the list (a multiple of 8)
+  bool far = false;  // a candidate 2^23 or more sorted positions away: no list
+  // group i / 8 of a's list starts at nbrGroupByte(a, 0) + (i / 8) * 1536 bytes
+  char* const list0 = reinterpret_cast<char*>(L.nbr) + nbrGroupByte(a, 0);
+  static_assert(nbrGroupByte(0, 1) == kNbrWaveGroupBytes, "groups of a particle 1536 bytes apart");
+  const uint32_t bias = kNbrBias - a;  // ring entries are stored biased: b - a + 2^23
+  auto storeGroup = [&](uint32_t first, uint32_t valid) {  // entries [first, first + 8)
+    // first is a multiple of 8: its slots are (first & 8) + 0..7, no wrap
+    const uint32_t* w = win + (first & kNbrGroup) * BS + threadIdx.x;
+    uint32_t e[kNbrGroup], q[6];
+#pragma unroll
+    for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = j < valid ? w[j * BS] : kNbrBias;
+    packGroup(e, q);
+    uint3* g = reinterpret_cast<uint3*>(list0 + (first / kNbrGroup) * kNbrWaveGroupBytes);
+    g[0] = make_uint3(q[0], q[1], q[2]);
+    g[1] = make_uint3(q[3], q[4], q[5]);
+  };
+  forEachNeighbourCellRolled(key, table, c.G, [&](uint32_t s, uint32_t e) {
+    // a cell's candidates are the run [s, e): its ends bound their distance
+    if (checkFar && ((s - a + kNbrBias) >> 24 || (e - 1u - a + kNbrBias) >> 24)) far = true;
+    for (uint32_t b0 = s; b0 < e; b0 += kBuildBlock) {
+      decltype(load(0u)) rec[kBuildBlock];  // candidates loaded together (indices clamped into the cell)
+#pragma unroll
+      for (int k = 0; k < kBuildBlock; ++k) rec[k] = load(min(b0 + k, e - 1u));
+#pragma unroll
+      for (int k = 0; k < kBuildBlock; ++k) {
+        const uint32_t b = b0 + k;
+        const float4& pb = posOf(rec[k]);
+        // both tests always evaluated (the load index is clamped into the
+        // cell): no branch per candidate
+        const uint32_t accept =
+            (b < e ? 1u : 0u) &
+            (dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z) <= c.listR2 ? 1u : 0u);
+        // written whether accepted or not: slot cnt % 16 is the next free
+        // one, so a rejected candidate is overwritten by the next accepted
+        win[(cnt % kRing) * BS + threadIdx.x] = b + bias;
+        cnt += accept;
+        visit(b, rec[k], b < e);
+      }
+      if (cnt - stored >= kNbrGroup) {
+        if (stored + kNbrGroup <= kMaxNbr) storeGroup(stored, kNbrGroup);
+        stored += kNbrGroup;
+      }
+    }
+  });
+  // The last cnt % 8 entries, and the list padded with a itself to whole
+  // blocks of kListPad entries: readers load whole blocks and skip the
+  // entries past the count, so every loaded index is a valid particle.
+  if (cnt <= kMaxNbr && !far) {
+    if (cnt != stored) storeGroup(stored, cnt - stored);
+    for (uint32_t s = (cnt + kNbrGroup - 1) / kNbrGroup * kNbrGroup; s % kListPad != 0u;
+         s += kNbrGroup)
+      storeGroup(s, 0);
+  }
+  L.count[a] = far ? kMaxNbr + 1u : cnt;
+}
+
+// Neighbour lists of the predicted p* (see the file comment), from P0 (the
+// clamped p*, written by the reorder kernel): epoch 1 for every particle.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __restrict__ keys,
                                                    const uint32_t* __restrict__ table,
                                                    const uint8_t* __restrict__ T, Lists L,
                                                    int checkFar) {
-  constexpr uint32_t kRing = 2 * kNbrGroup;
-  // at most 7 entries wait before a block, so at most 15 after it: one group
-  // completes at most, and a block's writes always find a free slot
-  static_assert(kBuildBlock <= int(kNbrGroup), "a block completes at most one group");
   // win[j * BS + lane] holds entry j mod 16 of the lane's list; lanes side by
   // side, so no bank conflicts
-  __shared__ uint32_t win[kRing * BS];
-  if (blockIdx.x == 0 && threadIdx.x < 4) L.maxDisp[threadIdx.x] = 0u;
+  __shared__ uint32_t win[2 * kNbrGroup * BS];
+  if (blockIdx.x == 0 && threadIdx.x < 4 + kPassSlots) L.maxDisp[threadIdx.x] = 0u;
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
     if (a >= ch.end) return;
+    L.epoch[a] = 1u;
     if (T[a] == LF_OBSTACLE) {
       L.count[a] = 0;
       return;
     }
-    const float4 pa = L.P0[a];
-    uint32_t cnt = 0, stored = 0;  // entries accepted / written to the list (a multiple of 8)
-    bool far = false;  // a candidate 2^23 or more sorted positions away: no list
-    // group i / 8 of a's list starts at nbrGroupByte(a, 0) + (i / 8) * 1536 bytes
-    char* const list0 = reinterpret_cast<char*>(L.nbr) + nbrGroupByte(a, 0);
-    static_assert(nbrGroupByte(0, 1) == kNbrWaveGroupBytes, "groups of a particle 1536 bytes apart");
-    const uint32_t bias = kNbrBias - a;  // ring entries are stored biased: b - a + 2^23
-    auto storeGroup = [&](uint32_t first, uint32_t valid) {  // entries [first, first + 8)
-      // first is a multiple of 8: its slots are (first & 8) + 0..7, no wrap
-      const uint32_t* w = win + (first & kNbrGroup) * BS + threadIdx.x;
-      uint32_t e[kNbrGroup], q[6];
-#pragma unroll
-      for (uint32_t j = 0; j < kNbrGroup; ++j) e[j] = j < valid ? w[j * BS] : kNbrBias;
-      packGroup(e, q);
-      uint3* g = reinterpret_cast<uint3*>(list0 + (first / kNbrGroup) * kNbrWaveGroupBytes);
-      g[0] = make_uint3(q[0], q[1], q[2]);
-      g[1] = make_uint3(q[3], q[4], q[5]);
-    };
-    forEachNeighbourCellRolled(keys[a], table + c.keyBase, c.G, [&](uint32_t s, uint32_t e) {
-      // a cell's candidates are the run [s, e): its ends bound their distance
-      if (checkFar && ((s - a + kNbrBias) >> 24 || (e - 1u - a + kNbrBias) >> 24)) far = true;
-      for (uint32_t b0 = s; b0 < e; b0 += kBuildBlock) {
-        float4 pb[kBuildBlock];  // candidates loaded together (indices clamped into the cell)
-#pragma unroll
-        for (int k = 0; k < kBuildBlock; ++k) pb[k] = at(L.P0, min(b0 + k, e - 1u));
-#pragma unroll
-        for (int k = 0; k < kBuildBlock; ++k) {
-          const uint32_t b = b0 + k;
-          // both tests always evaluated (the load index is clamped into the
-          // cell): no branch per candidate
-          const uint32_t accept =
-              (b < e ? 1u : 0u) &
-              (dist2(pa.x - pb[k].x, pa.y - pb[k].y, pa.z - pb[k].z) <= c.listR2 ? 1u : 0u);
-          // written whether accepted or not: slot cnt % 16 is the next free
-          // one, so a rejected candidate is overwritten by the next accepted
-          win[(cnt % kRing) * BS + threadIdx.x] = b + bias;
-          cnt += accept;
-        }
-        if (cnt - stored >= kNbrGroup) {
-          if (stored + kNbrGroup <= kMaxNbr) storeGroup(stored, kNbrGroup);
-          stored += kNbrGroup;
-        }
-      }
-    });
-    // The last cnt % 8 entries, and the list padded with a itself to whole
-    // blocks of kListPad entries: readers load whole blocks and skip the
-    // entries past the count, so every loaded index is a valid particle.
-    if (cnt <= kMaxNbr && !far) {
-      if (cnt != stored) storeGroup(stored, cnt - stored);
-      for (uint32_t s = (cnt + kNbrGroup - 1) / kNbrGroup * kNbrGroup;
-           s % kListPad != 0u; s += kNbrGroup)
-        storeGroup(s, 0);
-    }
-    L.count[a] = far ? kMaxNbr + 1u : cnt;
+    buildList<BS>(c, L, win, a, L.P0[a], keys[a], table + c.keyBase, checkFar,
+                  [&](uint32_t b) { return at(L.P0, b); }, [](uint32_t, const float4&, bool) {});
   });
 }
 
@@ -229,13 +284,6 @@ __device__ inline void lambdaTerm(const PbfConsts& c, float ma, float dx, float 
   }
 }
 
-// Gathered neighbour records: a float4 whose xyz is p*, or position and
-// velocity together (XSPH).
-struct PosVel {
-  float4 p, v;
-};
-__device__ inline const float4& posOf(const float4& r) { return r; }
-__device__ inline const float4& posOf(const PosVel& r) { return r.p; }
 
 // Entries i .. i+BLK-1 of particle a's list as particle indices: two 12-byte
 // loads per group of 8, unpacked, b = a + entry - 2^23.
@@ -261,10 +309,42 @@ __device__ inline void readListBlock(const Lists& L, uint32_t a, uint32_t i, uin
 // kWalkBlock at a time (indices clamped into the cell, so every load is a
 // valid particle) and used in order, with the next cell's table entries
 // loaded during the visit; one memory latency per block, not per candidate.
+// rel(record, dx, dy, dz) gives p_a - p_b from a gathered record: FP32
+// differences of the float4 p* (RelF32), or of the fixed-point coordinates
+// (RelQ, LF_FLAG_PACKED16); use(b, record, dx, dy, dz, d2) runs for the
+// accepted ones.
+struct RelF32 {
+  float4 pa;
+  template <typename R> __device__ void operator()(const R& r, float& dx, float& dy, float& dz) const {
+    const float4& q = posOf(r);
+    dx = pa.x - q.x;
+    dy = pa.y - q.y;
+    dz = pa.z - q.z;
+  }
+};
+struct QRec {  // a neighbour of the packed delta pass: its record and its lambda
+  uint2 q;
+  float lam;
+};
+__device__ inline uint2 qOf(const uint2& r) { return r; }
+__device__ inline uint2 qOf(const QRec& r) { return r.q; }
+__device__ inline uint2 qOf(const PosQ& r) { return r.q; }
+struct RelQ {
+  QPos qa;
+  float unit;
+  template <typename R> __device__ void operator()(const R& r, float& dx, float& dy, float& dz) const {
+    const QPos qb = unpackPos(qOf(r));
+    dx = qDiff(qa.x, qb.x, unit);
+    dy = qDiff(qa.y, qb.y, unit);
+    dz = qDiff(qa.z, qb.z, unit);
+  }
+};
+
 constexpr int kWalkBlock = 4;
-template <typename Load, typename Use>
-__device__ inline void walkCells(const PbfConsts& c, float4 pa, uint32_t key,
-                                 const uint32_t* __restrict__ table, Load&& load, Use&& use) {
+template <typename Load, typename Rel, typename Use>
+__device__ inline void walkCells(const PbfConsts& c, uint32_t key,
+                                 const uint32_t* __restrict__ table, Load&& load, Rel&& rel,
+                                 Use&& use) {
   forEachNeighbourCellRolled(key, table, c.G, [&](uint32_t s, uint32_t e) {
     for (uint32_t b0 = s; b0 < e; b0 += kWalkBlock) {
       decltype(load(0u)) p[kWalkBlock];
@@ -272,9 +352,10 @@ __device__ inline void walkCells(const PbfConsts& c, float4 pa, uint32_t key,
       for (int k = 0; k < kWalkBlock; ++k) p[k] = load(min(b0 + k, e - 1u));
 #pragma unroll
       for (int k = 0; k < kWalkBlock; ++k) {
-        const float4& q = posOf(p[k]);
-        const float d2 = dist2(pa.x - q.x, pa.y - q.y, pa.z - q.z);
-        if (b0 + k < e && d2 <= c.rmax2) use(b0 + k, p[k], d2);
+        float dx, dy, dz;
+        rel(p[k], dx, dy, dz);
+        const float d2 = dist2(dx, dy, dz);
+        if (b0 + k < e && d2 <= c.rmax2) use(b0 + k, p[k], dx, dy, dz, d2);
       }
     }
   });
@@ -287,14 +368,26 @@ __device__ inline void walkCells(const PbfConsts& c, float4 pa, uint32_t key,
 // terms in list order, so a block costs one memory latency instead of BLK.
 // Entries past the count (the tail of the last block) hold a itself (the
 // build pads every list to whole blocks), a valid address, and are skipped.
-template <int BLK, typename Load, typename Use>
+//
+// Rebuild (the lambda passes after the first, win non-null): a wave that
+// would walk its cells rebuilds its lanes' lists from this pass's positions
+// in the same walk (buildList over the records of loadPos, which gives p* and
+// the pass's record), evaluating the pair terms as it goes, and marks them
+// epoch passIdx. The delta pass of the same iteration reads the same
+// positions, so it walks the new lists, and later passes keep them while
+// listUsable allows. The accepted neighbours and their order are the cell
+// walk's.
+template <int BS, int BLK, bool REBUILD, typename Load, typename Rel, typename Use,
+          typename LoadPos>
 __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int pass, bool fresh,
-                                       uint32_t a, float4 pa, uint32_t key,
-                                       const uint32_t* __restrict__ table, Load&& load,
-                                       Use&& use) {
-  uint32_t cnt;
-  bool usable = listUsable(c, L, a, pa.x, pa.y, pa.z, fresh, cnt);
-  if (!usable && cnt <= kMaxNbr) usable = listUsableLocal(c, L, a, pa.x, pa.y, pa.z, key);
+                                       int passIdx, uint32_t a, float4 pa, uint32_t key,
+                                       const uint32_t* __restrict__ table, Load&& load, Rel&& rel,
+                                       Use&& use, uint32_t* __restrict__ win, int checkFar,
+                                       LoadPos&& loadPos) {
+  uint32_t cnt, ep;
+  bool usable = listUsable(c, L, a, pa.x, pa.y, pa.z, fresh, passIdx, cnt, ep);
+  if (!usable && cnt <= kMaxNbr && (ep == 1u || passIdx < kPassSlots))
+    usable = listUsableLocal(c, L, a, pa.x, pa.y, pa.z, key, ep);
   const uint64_t fell = __ballot(!usable);
   if (fell && __lane_id() == __builtin_ctzll(fell)) atomicAdd(&L.maxDisp[pass], uint32_t(__popcll(fell)));
   // A wave with any lane walking its cells walks cells on every lane: the
@@ -302,6 +395,20 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
   // cost more than the walk alone. The accepted neighbours and their order
   // are the same either way.
   if (fell) usable = false;
+  if constexpr (REBUILD) {
+    if (!usable && win && passIdx < kPassSlots) {
+      buildList<BS>(c, L, win, a, pa, key, table, checkFar, loadPos,
+                    [&](uint32_t b, const auto& r, bool valid) {
+                      float dx, dy, dz;
+                      rel(r, dx, dy, dz);
+                      const float d2 = dist2(dx, dy, dz);
+                      if (valid && d2 <= c.rmax2) use(b, r, dx, dy, dz, d2);
+                    });
+      L.epoch[a] = uint8_t(passIdx);
+      L.R[a] = make_float4(pa.x, pa.y, pa.z, 0.0f);
+      return;
+    }
+  }
   if (usable) {
     static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
     static_assert(kListPad % BLK == 0, "the build pads lists to whole blocks");
@@ -313,13 +420,14 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
       for (int k = 0; k < BLK; ++k) p[k] = load(b[k]);
 #pragma unroll
       for (int k = 0; k < BLK; ++k) {
-        const float4& q = posOf(p[k]);
-        const float d2 = dist2(pa.x - q.x, pa.y - q.y, pa.z - q.z);
-        if (i + k < cnt && d2 <= c.rmax2) use(b[k], p[k], d2);
+        float dx, dy, dz;
+        rel(p[k], dx, dy, dz);
+        const float d2 = dist2(dx, dy, dz);
+        if (i + k < cnt && d2 <= c.rmax2) use(b[k], p[k], dx, dy, dz, d2);
       }
     }
   } else {
-    walkCells(c, pa, key, table, load, use);
+    walkCells(c, key, table, load, rel, use);
   }
 }
 
@@ -332,13 +440,14 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
 // use(p_b, v_b, d2) in list order. Same neighbours, same order.
 template <int BLK, typename Use>
 __device__ inline void forEachAcceptedSplit(const PbfConsts& c, const Lists& L, int pass,
-                                            uint32_t a, float4 pa, uint32_t key,
+                                            int passIdx, uint32_t a, float4 pa, uint32_t key,
                                             const uint32_t* __restrict__ table,
                                             const float4* __restrict__ P,
                                             const float4* __restrict__ V, Use&& use) {
-  uint32_t cnt;
-  bool usable = listUsable(c, L, a, pa.x, pa.y, pa.z, false, cnt);
-  if (!usable && cnt <= kMaxNbr) usable = listUsableLocal(c, L, a, pa.x, pa.y, pa.z, key);
+  uint32_t cnt, ep;
+  bool usable = listUsable(c, L, a, pa.x, pa.y, pa.z, false, passIdx, cnt, ep);
+  if (!usable && cnt <= kMaxNbr && (ep == 1u || passIdx < kPassSlots))
+    usable = listUsableLocal(c, L, a, pa.x, pa.y, pa.z, key, ep);
   const uint64_t fell = __ballot(!usable);
   if (fell && __lane_id() == __builtin_ctzll(fell)) atomicAdd(&L.maxDisp[pass], uint32_t(__popcll(fell)));
   if (fell) usable = false;  // the whole wave walks cells (as forEachAccepted)
@@ -367,36 +476,57 @@ __device__ inline void forEachAcceptedSplit(const PbfConsts& c, const Lists& L, 
     }
   } else {
     walkCells(
-        c, pa, key, table, [&](uint32_t b) { return PosVel{at(P, b), at(V, b)}; },
-        [&](uint32_t, const PosVel& r, float d2) { use(r.p, r.v, d2); });
+        c, key, table, [&](uint32_t b) { return PosVel{at(P, b), at(V, b)}; }, RelF32{pa},
+        [&](uint32_t, const PosVel& r, float, float, float, float d2) { use(r.p, r.v, d2); });
   }
 }
 
 // sph_lambda (oclsph_kernel.h:305-332). Rejected candidates contribute
 // exactly zero in the reference, so walking only the accepted ones gives the
 // same sums in the same order.
-template <int BS, bool KAHAN, bool FAST>
+// PACKED (LF_FLAG_PACKED16): neighbours' positions from the fixed-point
+// records Q; the pass copies each record to QL and lambda to LAM for the
+// delta pass (as PL for the FP32 gathers).
+template <int BS, bool KAHAN, bool FAST, bool PACKED>
 __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restrict__ keys,
                                                const uint32_t* __restrict__ table,
                                                const uint8_t* __restrict__ T,
                                                const float4* __restrict__ P,
                                                float4* __restrict__ PL, float* __restrict__ rhoOut,
-                                               Lists L, int fresh) {
+                                               Lists L, int fresh, int passIdx, int rebuild,
+                                               int checkFar, const uint2* __restrict__ Q,
+                                               uint2* __restrict__ QL, float* __restrict__ LAM) {
+  __shared__ uint32_t win[2 * kNbrGroup * BS];  // list rebuilds (forEachAccepted)
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
     if (a >= ch.end) return;
     const float4 pa = P[a];
+    uint2 qa = make_uint2(0u, 0u);
+    if (PACKED) {
+      qa = Q[a];
+      QL[a] = qa;
+    }
     if (T[a] == LF_OBSTACLE) {
       PL[a] = make_float4(pa.x, pa.y, pa.z, 0.0f);
       rhoOut[a] = 0.0f;
+      if (PACKED) LAM[a] = 0.0f;
       return;
     }
     LambdaSums A;
-    forEachAccepted<kBlockLambda>(
-        c, L, 1, fresh != 0, a, pa, keys[a], table + c.keyBase, [&](uint32_t b) { return at(P, b); },
-        [&](uint32_t, float4 pb, float d2) {
-          lambdaTerm<KAHAN, FAST>(c, pa.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, A);
-        });
+    auto term = [&](uint32_t, const auto&, float dx, float dy, float dz, float d2) {
+      lambdaTerm<KAHAN, FAST>(c, pa.w, dx, dy, dz, d2, A);
+    };
+    uint32_t* const ring = rebuild ? win : nullptr;
+    if (PACKED)
+      forEachAccepted<BS, kBlockLambda, true>(
+          c, L, 1, fresh != 0, passIdx, a, pa, keys[a], table + c.keyBase,
+          [&](uint32_t b) { return at(Q, b); }, RelQ{unpackPos(qa), c.qUnit}, term, ring,
+          checkFar, [&](uint32_t b) { return PosQ{at(P, b), at(Q, b)}; });
+    else
+      forEachAccepted<BS, kBlockLambda, true>(
+          c, L, 1, fresh != 0, passIdx, a, pa, keys[a], table + c.keyBase,
+          [&](uint32_t b) { return at(P, b); }, RelF32{pa}, term, ring, checkFar,
+          [&](uint32_t b) { return at(P, b); });
     if (KAHAN) {
       A.gx += A.cgx;
       A.gy += A.cgy;
@@ -405,8 +535,10 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
     }
     const float norm2 = dist2(A.gx, A.gy, A.gz);
     const float C1 = A.rho / c.rho0 - 1.0f;
-    PL[a] = make_float4(pa.x, pa.y, pa.z, -C1 / (norm2 + c.cfm));
+    const float lam = -C1 / (norm2 + c.cfm);
+    PL[a] = make_float4(pa.x, pa.y, pa.z, lam);
     rhoOut[a] = A.rho;
+    if (PACKED) LAM[a] = lam;
   });
 }
 
@@ -432,7 +564,7 @@ __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float d
 // FINAL (the last pass of a step whose passes all run): also does
 // k_finalise's work for the particle, from the same values in the same
 // order, so the separate finalise launch and its pass over the state go.
-template <int BS, bool FAST, bool FINAL>
+template <int BS, bool FAST, bool FINAL, bool PACKED>
 __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict__ keys,
                                               const uint32_t* __restrict__ table,
                                               const uint8_t* __restrict__ T,
@@ -441,27 +573,43 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
                                               float4* __restrict__ Pout, float4* __restrict__ V,
                                               ColliderGrid mesh,
                                               const uint32_t* __restrict__ diffused,
-                                              int useDiffused) {
-  __shared__ float waveMax[BS / 64];
+                                              int useDiffused, const uint2* __restrict__ QL,
+                                              const float* __restrict__ LAM,
+                                              uint2* __restrict__ Qout, int passIdx) {
+  __shared__ float waveMax[BS / 64], waveMove[BS / 64];
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
     const bool valid = a < ch.end;
-    float disp = 0.0f;
+    float disp = 0.0f;  // |p* - P0|, since the step's list build
+    float move = 0.0f;  // |p* - p* of this pass's input|: this pass's move
     if (valid) {
       const float4 pa = PL[a];
       const float mass = X[a].w;
       if (T[a] == LF_OBSTACLE) {
         // finalise leaves obstacles in place with P.w = 0 (no fluid mass)
         Pout[a] = make_float4(pa.x, pa.y, pa.z, FINAL ? 0.0f : mass);
+        if (PACKED) Qout[a] = QL[a];
       } else {
         const float la = pa.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        forEachAccepted<kBlockDelta>(
-            c, L, 2, fresh != 0, a, pa, keys[a], table + c.keyBase,
-            [&](uint32_t b) { return at(PL, b); },
-            [&](uint32_t, float4 pb, float d2) {
-              deltaTerm<FAST>(c, la, pb.w, pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d2, ax, ay, az);
-            });
+        auto noRebuild = [](uint32_t) { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); };
+        if (PACKED)
+          forEachAccepted<BS, kBlockDelta, false>(
+              c, L, 2, fresh != 0, passIdx, a, pa, keys[a], table + c.keyBase,
+              [&](uint32_t b) { return QRec{at(QL, b), at(LAM, b)}; },
+              RelQ{unpackPos(QL[a]), c.qUnit},
+              [&](uint32_t, const QRec& r, float dx, float dy, float dz, float d2) {
+                deltaTerm<FAST>(c, la, r.lam, dx, dy, dz, d2, ax, ay, az);
+              },
+              nullptr, 0, noRebuild);
+        else
+          forEachAccepted<BS, kBlockDelta, false>(
+              c, L, 2, fresh != 0, passIdx, a, pa, keys[a], table + c.keyBase,
+              [&](uint32_t b) { return at(PL, b); }, RelF32{pa},
+              [&](uint32_t, const float4& pb, float dx, float dy, float dz, float d2) {
+                deltaTerm<FAST>(c, la, pb.w, dx, dy, dz, d2, ax, ay, az);
+              },
+              nullptr, 0, noRebuild);
         // pos = (p* + dp) * scale, mesh colliders (:364-368), clamped to the
         // bounds (:371), p* = pos/scale (:374)
         const float acc[3] = {ax, ay, az};
@@ -483,6 +631,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
           o[k] = q / c.scale;
         }
         Pout[a] = make_float4(o[0], o[1], o[2], mass);
+        if (PACKED) Qout[a] = packPos(c, o[0], o[1], o[2]);
         if (FINAL) {  // k_finalise on p* = o; V[a] after the collider response
           const float4 x = X[a];
           const float4 v = V[a];
@@ -498,6 +647,8 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
         const float4 p0 = L.P0[a];
         disp = __builtin_sqrtf(dist2(o[0] - p0.x, o[1] - p0.y, o[2] - p0.z));
         if (!(disp < 3.0e38f)) disp = 3.0e38f;  // NaN / inf: every list unusable
+        move = __builtin_sqrtf(dist2(o[0] - pa.x, o[1] - pa.y, o[2] - pa.z));
+        if (!(move < 3.0e38f)) move = 3.0e38f;
       }
     }
     // per-cell maximum: one atomic per run of equal keys within the wave
@@ -514,18 +665,32 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
     if (valid && head && runMax > 0.0f && key + 1u < c.G)
       atomicMax(&L.cellDisp[c.keyBase + key], __float_as_uint(runMax));
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
-    // D: the workgroup's maximum, then one atomic, and only when it raises D
-    // (D only grows during the pass, so a value read here is a lower bound)
-    if (__lane_id() == 0) waveMax[threadIdx.x >> 6] = disp;
+    for (int off = 32; off > 0; off >>= 1) {
+      disp = fmaxf(disp, __shfl_xor(disp, off, 64));
+      move = fmaxf(move, __shfl_xor(move, off, 64));
+    }
+    // D and D_j: the workgroup's maxima, then one atomic each, and only when
+    // it raises the value (they only grow during the pass, so a value read
+    // here is a lower bound)
+    if (__lane_id() == 0) {
+      waveMax[threadIdx.x >> 6] = disp;
+      waveMove[threadIdx.x >> 6] = move;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-      float m = 0.0f;
+      float m = 0.0f, mv = 0.0f;
 #pragma unroll
-      for (int w = 0; w < BS / 64; ++w) m = fmaxf(m, waveMax[w]);
+      for (int w = 0; w < BS / 64; ++w) {
+        m = fmaxf(m, waveMax[w]);
+        mv = fmaxf(mv, waveMove[w]);
+      }
       const uint32_t bits = __float_as_uint(m);
       if (m > 0.0f && __hip_atomic_load(L.maxDisp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bits)
         atomicMax(L.maxDisp, bits);
+      uint32_t* const dj = L.maxDisp + 4 + min(passIdx, kPassSlots - 1);
+      const uint32_t mbits = __float_as_uint(mv);
+      if (mv > 0.0f && __hip_atomic_load(dj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < mbits)
+        atomicMax(dj, mbits);
     }
     __syncthreads();
   });
@@ -626,12 +791,15 @@ __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict_
                                              const uint8_t* __restrict__ T,
                                              const float4* __restrict__ P,
                                              const float4* __restrict__ Vin, Lists L,
-                                             float4* __restrict__ Vout) {
+                                             float4* __restrict__ Vout,
+                                             const uint32_t* __restrict__ health, int passIdx) {
+  // a diverged state: velocities copied through (Vout becomes the current V)
+  const bool halted = (health[0] | health[1]) != 0u;
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t a = ch.begin + threadIdx.x;
     if (a >= ch.end) return;
     const float4 va = Vin[a];
-    if (T[a] == LF_OBSTACLE || c.xsph == 0.0f) {
+    if (halted || T[a] == LF_OBSTACLE || c.xsph == 0.0f) {
       Vout[a] = va;
       return;
     }
@@ -650,13 +818,14 @@ __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict_
     // List walk in two gathers per block: positions, then the velocities of
     // the entries within h only (about half the list lies in the skin).
     forEachAcceptedSplit<kBlockXsph>(
-        c, L, 3, a, pa, keys[a], table + c.keyBase, P, Vin,
+        c, L, 3, passIdx, a, pa, keys[a], table + c.keyBase, P, Vin,
         [&](const float4& pb, const float4& vb, float d2) { term(pb, vb, d2); });
 #else
-    forEachAccepted<kBlockXsph>(
-        c, L, 3, false, a, pa, keys[a], table + c.keyBase,
-        [&](uint32_t b) { return PosVel{at(P, b), at(Vin, b)}; },
-        [&](uint32_t, const PosVel& r, float d2) { term(r.p, r.v, d2); });
+    forEachAccepted<BS, kBlockXsph, false>(
+        c, L, 3, false, passIdx, a, pa, keys[a], table + c.keyBase,
+        [&](uint32_t b) { return PosVel{at(P, b), at(Vin, b)}; }, RelF32{pa},
+        [&](uint32_t, const PosVel& r, float, float, float, float d2) { term(r.p, r.v, d2); },
+        nullptr, 0, [&](uint32_t b) { return PosVel{at(P, b), at(Vin, b)}; });
 #endif
     Vout[a] = make_float4(__builtin_fmaf(c.xsphRho, ax, va.x),
                           __builtin_fmaf(c.xsphRho, ay, va.y),
@@ -665,7 +834,15 @@ __global__ __launch_bounds__(BS) void k_xsph(Work W, const uint32_t* __restrict_
 }
 
 Lists listsOf(Ctx* c) {
-  return Lists{c->nbr.p, c->nbrCount.p, c->P0.p, c->maxDisp.p, c->cellDisp.p};
+  return Lists{c->nbr.p, c->nbrCount.p, c->P0.p, c->maxDisp.p, c->cellDisp.p, c->listEpoch.p,
+               c->listR.p};
+}
+
+// The work of a pass that stops once the state has diverged (health counters).
+Work haltingWork(Ctx* c, int queueSlot, bool skipDone = false) {
+  Work W = workOf(c, queueSlot, skipDone);
+  W.halt = c->health.p;
+  return W;
 }
 
 }  // namespace
@@ -673,7 +850,8 @@ Lists listsOf(Ctx* c) {
 int pbfBuildLists(Ctx* c) {
   if (c->n == 0) return LF_OK;
   c->listFresh = true;
-  const Work W = workOf(c, kSlotList);
+  c->passIdx = 1;
+  const Work W = haltingWork(c, kSlotList);
   return LF_DISPATCH_BS(c, "pbf_build_list", k_build_list, W,
                         (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
                         (const uint8_t*)c->T[c->cur].p, listsOf(c),
@@ -693,23 +871,36 @@ template <typename F> int withFlag(bool on, F&& f) {
   return on ? f(std::true_type{}) : f(std::false_type{});
 }
 
+// The packed records exist only when the grid was built with the flag.
+static int requirePacked(Ctx* c, bool packed) {
+  if (packed && !c->gridPacked)
+    return fail(c, LF_ERR_STATE, "LF_FLAG_PACKED16: build the grid with the flag set");
+  return LF_OK;
+}
+
 int pbfLambda(Ctx* c, bool skipDone) {
   if (c->n == 0) return LF_OK;
-  const Work W = workOf(c, kSlotLambda, skipDone);
+  if (int rc = requirePacked(c, (c->consts.flags & LF_FLAG_PACKED16) != 0)) return rc;
+  const Work W = haltingWork(c, kSlotLambda, skipDone);
   const int fresh = int(c->listFresh);
   const uint32_t flags = c->consts.flags;
   const char* name = (flags & LF_FLAG_KAHAN)
                          ? ((flags & LF_FLAG_FAST_MATH) ? "pbf_lambda_kahan_fast" : "pbf_lambda_kahan")
                          : ((flags & LF_FLAG_FAST_MATH) ? "pbf_lambda_fast" : "pbf_lambda");
+  const bool packed = (flags & LF_FLAG_PACKED16) != 0;
   return withBS(c, [&](auto bs) {
     return withFlag(flags & LF_FLAG_KAHAN, [&](auto kahan) {
       return withFlag(flags & LF_FLAG_FAST_MATH, [&](auto fast) {
-        return launch(c, name, k_lambda<decltype(bs)::value, decltype(kahan)::value,
-                                        decltype(fast)::value>,
-                      gridOf(c, W), dim3(decltype(bs)::value), 0, W,
-                      (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
-                      (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p, c->rho.p,
-                      listsOf(c), fresh);
+        return withFlag(packed, [&](auto pk) {
+          return launch(c, name, k_lambda<decltype(bs)::value, decltype(kahan)::value,
+                                          decltype(fast)::value, decltype(pk)::value>,
+                        gridOf(c, W), dim3(decltype(bs)::value), 0, W,
+                        (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
+                        (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p,
+                        c->rho.p, listsOf(c), fresh, c->passIdx, int(c->passIdx > 1),
+                        int(c->n > size_t(kNbrBias) + 1), (const uint2*)c->Q.p, c->QL.p,
+                        c->LAM.p);
+        });
       });
     });
   });
@@ -717,34 +908,44 @@ int pbfLambda(Ctx* c, bool skipDone) {
 
 int pbfDelta(Ctx* c, bool skipDone, bool final) {
   if (c->n == 0) return LF_OK;
+  if (int rc = requirePacked(c, (c->consts.flags & LF_FLAG_PACKED16) != 0)) return rc;
   int grc = prepareColliderGrid(c);
   if (grc) return grc;
-  const Work W = workOf(c, kSlotDelta, skipDone);
+  if ((grc = waitPairReaders(c))) return grc;
+  const Work W = haltingWork(c, kSlotDelta, skipDone);
   const bool fastMath = (c->consts.flags & LF_FLAG_FAST_MATH) != 0;
   const char* name = final ? (fastMath ? "pbf_delta_final_fast" : "pbf_delta_final")
                            : (fastMath ? "pbf_delta_fast" : "pbf_delta");
+  const bool packed = (c->consts.flags & LF_FLAG_PACKED16) != 0;
   const int rc = withBS(c, [&](auto bs) {
     return withFlag(fastMath, [&](auto fast) {
       return withFlag(final, [&](auto fin) {
-        return launch(c, name,
-                      k_delta<decltype(bs)::value, decltype(fast)::value, decltype(fin)::value>,
-                      gridOf(c, W), dim3(decltype(bs)::value), 0, W,
-                      (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
-                      (const uint8_t*)c->T[c->cur].p, (const float4*)c->PL.p, c->X[c->cur].p,
-                      listsOf(c), int(c->listFresh), c->P.p, c->V[c->cur].p, c->colliderGrid,
-                      (const uint32_t*)c->diffused.p,
-                      int((c->consts.flags & LF_FLAG_DIFFUSE) != 0));
+        return withFlag(packed, [&](auto pk) {
+          return launch(c, name,
+                        k_delta<decltype(bs)::value, decltype(fast)::value, decltype(fin)::value,
+                                decltype(pk)::value>,
+                        gridOf(c, W), dim3(decltype(bs)::value), 0, W,
+                        (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
+                        (const uint8_t*)c->T[c->cur].p, (const float4*)c->PL.p, c->X[c->cur].p,
+                        listsOf(c), int(c->listFresh), c->P.p, c->V[c->cur].p, c->colliderGrid,
+                        (const uint32_t*)c->diffused.p,
+                        int((c->consts.flags & LF_FLAG_DIFFUSE) != 0), (const uint2*)c->QL.p,
+                        (const float*)c->LAM.p, c->Q.p, c->passIdx);
+        });
       });
     });
   });
   c->listFresh = false;  // p* now differs from the predicted positions
+  ++c->passIdx;
   if (rc == LF_OK) notePositionsWritten(c);
   return rc;
 }
 
 int pbfFinalise(Ctx* c) {
   if (c->n == 0) return LF_OK;
-  const Work W = workOf(c, -1);
+  int rc0 = waitPairReaders(c);
+  if (rc0) return rc0;
+  const Work W = haltingWork(c, -1);
   const int rc = LF_DISPATCH_BS(c, "pbf_finalise", k_finalise, W, (const uint8_t*)c->T[c->cur].p,
                                 c->P.p, c->X[c->cur].p, c->V[c->cur].p,
                                 (const uint32_t*)c->diffused.p,
@@ -765,7 +966,7 @@ int pbfXsph(Ctx* c) {
   rc = LF_DISPATCH_BS(c, "pbf_xsph", k_xsph, W, (const uint32_t*)c->localKey.p,
                       (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
                       (const float4*)c->P.p, (const float4*)c->V[c->cur].p, listsOf(c),
-                      c->V[nxt].p);
+                      c->V[nxt].p, (const uint32_t*)c->health.p, c->passIdx);
   if (rc) return rc;
   std::swap(c->V[0], c->V[1]);  // the XSPH output becomes the current V
   return LF_OK;
@@ -773,7 +974,7 @@ int pbfXsph(Ctx* c) {
 
 int pbfDiffuse(Ctx* c) {
   if (c->n == 0) return LF_OK;
-  const Work W = workOf(c, -1);
+  const Work W = haltingWork(c, -1);
   return LF_DISPATCH_BS(c, "pbf_diffuse", k_diffuse, W, (const uint32_t*)c->localKey.p,
                         (const uint32_t*)c->table.p, (const uint8_t*)c->T[c->cur].p,
                         (const float4*)c->V[c->cur].p, c->diffused.p);

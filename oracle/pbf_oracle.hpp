@@ -29,8 +29,9 @@
  *    cut (oclsph_kernel.h:150-192);
  *  - Jacobi delta pass (the reference updates p* in place while other
  *    work-items read it, oclsph_kernel.h:355,374: a data race);
- *  - stable sort by (key, caller index) (the reference's std::sort is
- *    unstable, oclsph.hpp:735-738).
+ *  - stable sort by (min(key, G), caller index): the reference's std::sort
+ *    is unstable (oclsph.hpp:735-738), and keys >= G belong to no cell
+ *    (oclsph_kernel.h:186-187), so they share one last bucket in caller order.
  *
  * Parity pins (tests/test_oracle.py): Morton encode/decode against the
  * reference's own curves.h built in oracle/_ref and against a bit-loop
@@ -237,8 +238,9 @@ template <typename F> class Pbf {
     }
     return false;
   }
-  /* sorted state of the current step */
-  std::vector<uint32_t> keyIn, keys, order, table;
+  /* sorted state of the current step: keys = sort keys min(key, G), cellKeys
+   * = the exact Morton keys the 27-cell walks decode */
+  std::vector<uint32_t> keyIn, keys, cellKeys, order, table;
   std::vector<F> ps, psNext, xs, vs, ms, lambda, rho;
   std::vector<int32_t> ts;
   std::vector<uint32_t> cs, diffused;
@@ -301,9 +303,11 @@ template <typename F> class Pbf {
     }
     order.resize(n);
     std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return keyIn[a] < keyIn[b]; });
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      return std::min(keyIn[a], c.G) < std::min(keyIn[b], c.G);
+    });
     keys.resize(n);
+    cellKeys.resize(n);
     ps.resize(4 * n);
     xs.resize(3 * n);
     vs.resize(3 * n);
@@ -313,7 +317,8 @@ template <typename F> class Pbf {
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < long(n); ++s) {
       const uint32_t i = order[s];
-      keys[s] = keyIn[i];
+      keys[s] = std::min(keyIn[i], c.G);
+      cellKeys[s] = keyIn[i];
       for (int k = 0; k < 3; ++k) {
         ps[4 * s + k] = pIn[3 * i + k];
         xs[3 * s + k] = x[3 * i + k];
@@ -365,7 +370,7 @@ template <typename F> class Pbf {
       const F* pa = &ps[4 * a];
       const F ma = ps[4 * a + 3];
       F g[3] = {0, 0, 0}, gc[3] = {0, 0, 0}, r = 0, rc = 0;
-      forEachCandidate(keys[a], [&](uint32_t b) {
+      forEachCandidate(cellKeys[a], [&](uint32_t b) {
         const F* pb = &ps[4 * b];
         const F dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
         const F d = std::sqrt(dist2(dx, dy, dz));
@@ -413,7 +418,7 @@ template <typename F> class Pbf {
       }
       const F la = lambda[a];
       F acc[3] = {0, 0, 0};
-      forEachCandidate(keys[a], [&](uint32_t b) {
+      forEachCandidate(cellKeys[a], [&](uint32_t b) {
         const F* pb = &ps[4 * b];
         const F dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
         const F d = std::sqrt(dist2(dx, dy, dz));
@@ -466,7 +471,7 @@ template <typename F> class Pbf {
       const F* pa = &ps[4 * a];
       const F* va = &vs[3 * a];
       F acc[3] = {0, 0, 0};
-      forEachCandidate(keys[a], [&](uint32_t b) {
+      forEachCandidate(cellKeys[a], [&](uint32_t b) {
         const F* pb = &ps[4 * b];
         const F dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
         const F d2 = dist2(dx, dy, dz);
@@ -491,7 +496,7 @@ template <typename F> class Pbf {
       if (ts[s] == LF_OBSTACLE) continue;
       F mix[4] = {0, 0, 0, 0};
       int N = 0;
-      forEachCandidate(keys[s], [&](uint32_t b) {
+      forEachCandidate(cellKeys[s], [&](uint32_t b) {
         if (ts[b] == LF_OBSTACLE) return;
         F cb[4];
         unpackARGB(cs[b], cb);

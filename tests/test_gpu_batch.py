@@ -113,16 +113,17 @@ def test_batch_keys_and_tables(ctx):
     assert table[0] == 0 and np.all(np.diff(table.astype(np.int64)) >= 0)
 
 
-@pytest.mark.parametrize("threads,persistent,skin", [
-    (128, False, 0.2), (512, False, 0.2), (256, True, 0.2), (128, True, 0.1), (256, False, 0.0),
-    (512, True, 0.3)])
-def test_exec_configs_same_results(ctx, threads, persistent, skin):
+@pytest.mark.parametrize("threads,persistent,skin,ppt", [
+    (128, False, 0.2, 1), (512, False, 0.2, 1), (256, True, 0.2, 1), (128, True, 0.1, 1),
+    (256, False, 0.0, 1), (512, True, 0.3, 1), (128, False, 0.2, 2), (256, False, 0.3, 4),
+    (512, False, 0.1, 4)])
+def test_exec_configs_same_results(ctx, threads, persistent, skin, ppt):
     sc = _small_batch(5, seed=13)
     _upload_batch(ctx, sc)
     ctx.batch_step(2)
     ref = _batch_states(ctx, sc)
     with lf.Context(0) as c:
-        c.exec_config(threads, persistent, skin)
+        c.exec_config(threads, persistent, skin, particles_per_thread=ppt)
         _upload_batch(c, sc)
         c.batch_step(2)
         got = _batch_states(c, sc)
@@ -207,7 +208,8 @@ def test_autotune_keeps_state():
         c.step(p, 1)
         before = c.download()
         r = c.autotune(2)
-        assert r["trials"] == 18 and r["threads"] in (128, 256, 512) and r["ms_per_step"] > 0
+        assert r["trials"] == 36 and r["threads"] in (128, 256, 512) and r["ms_per_step"] > 0
+        assert r["particles_per_thread"] in (1, 2, 4) and r["mc_tile"] == 0 and r["mc_ms"] == 0
         after = c.download()
         for k in ("position", "velocity", "colour", "order"):
             _same(after[k], before[k], "autotune changed %s" % k)
@@ -247,3 +249,26 @@ def test_batch_rejects_mixed_iterations(ctx):
     with pytest.raises(lf.LfError):
         ctx.batch_upload(ps, [dict(position=s.position, velocity=s.velocity, mass=s.mass)
                               for s in sc])
+
+
+def test_density_tile_shapes_same_field():
+    """The density field's sample tile (a runtime choice) changes nothing: the
+    same terms in the same order per sample, whether staged in LDS or read from
+    global memory; the autotuner times the shapes on the last density lattice."""
+    sc = scenes.dam_break_8k()
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
+    m = sc.mc()
+    fields = []
+    for tile in (0, 1, 2):
+        with lf.Context(0) as c:
+            c.exec_config(256, False, 0.2, mc_tile=tile)
+            c.upload(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
+            c.step(p, 2)
+            c.mc_sample(p, m)
+            fields.append(c.read(LF_ARR_FIELD))
+            if tile == 2:
+                r = c.autotune(1)
+                assert r["trials"] == 39 and r["mc_tile"] in (0, 1, 2) and r["mc_ms"] > 0
+    assert (fields[0] > 0).sum() > 1000
+    _same(fields[1], fields[0], "tile 4x8x8")
+    _same(fields[2], fields[0], "tile 8x4x8")

@@ -154,6 +154,24 @@ def test_step_1m_bit_exact(ctx):
     _eq(field.astype(np.float64).reshape(ofield.shape), ofield, "density field 1M 256^3")
 
 
+def test_steps_1m_collapse_bit_exact(ctx):
+    """The first 12 steps of the 1M dam break, where the constraint passes move
+    particles farther than the neighbour lists' skin and many walks fall back
+    to the 27 cells: bit for bit against the oracle's steps."""
+    sc = scenes.dam_break_1m()
+    p = sc.params()
+    o = _pair(ctx, sc, p)
+    fell = 0
+    for _ in range(12):
+        ctx.step(p, 1)
+        fell += int(ctx.read(LF_ARR_LISTSTATS)[1:].astype(np.int64).sum())
+    o.step(12)
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position 1M, 12 steps")
+    _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity 1M, 12 steps")
+    assert fell > 10000, "the walks did not fall back to the cells (%d)" % fell
+
+
 def test_far_neighbours_fall_back_bit_exact(ctx):
     """List entries are 24-bit offsets from the particle's own sorted position;
     a particle with a candidate 2^23 or more positions away (possible only in a
@@ -349,13 +367,16 @@ def test_morton_extent_error(ctx):
     assert e.value.code == LF_ERR_MORTON
 
 
-def test_sort_properties_random_keys(ctx):
+@pytest.mark.parametrize("h", [0.01, 0.1])
+def test_sort_properties_random_keys(ctx, h):
     """Size-independent properties on a large random cloud: permutation,
-    sortedness, stability, table = lower bound."""
+    sortedness, stability, table = lower bound. h = 0.01: ~2 particles per
+    cell; h = 0.1: ~700 per cell, every bucket ordered by the large-bucket sort."""
     rng = np.random.default_rng(11)
     n = 2_000_000
     pos = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
-    p = lf.pbf_params(0.01, gravity=(0, 0, 0), max_bound=(1, 1, 1))
+    p = lf.pbf_params(h, gravity=(0, 0, 0), max_bound=(1, 1, 1))
+    ctx.set_guard(0)
     ctx.upload(pos, np.zeros_like(pos), np.ones(n, np.float32))
     ctx.grid_build(p)
     keys = ctx.read(LF_ARR_KEYS)
@@ -368,6 +389,7 @@ def test_sort_properties_random_keys(ctx):
     G = table.shape[0]
     assert np.array_equal(table, np.searchsorted(keys, np.arange(G, dtype=np.uint32),
                                                  side="left").astype(np.uint32))
+    ctx.set_guard(256)
 
 
 def test_1m_hundred_steps_stay_physical(ctx):

@@ -1,10 +1,13 @@
 """GPU: the mixed-precision options and the FP64 diagnostics.
 
 * LF_FLAG_FAST_MATH (hardware sqrt / reciprocal in the pair terms) with and
-  without LF_FLAG_KAHAN, against the FP64 oracle on the same inputs, one step
-  (teacher forced): density relative error <= 1e-4, lambda error <= 1e-4 of
-  max|lambda|, position error <= 1e-4 h. The plain FP32 path is held to the
-  same bar, and FAST_MATH must not be far worse than plain FP32.
+  without LF_FLAG_KAHAN, and LF_FLAG_PACKED16 (neighbour positions as 16-bit
+  cell-relative fixed point), against the FP64 oracle on the same inputs, one
+  step (teacher forced): density relative error <= 1e-4, lambda error <= 1e-4
+  of max|lambda| (5e-4 with PACKED16: one quantum, h/65536, per coordinate on
+  top of the FP32 rounding of the inputs), position error <= 1e-4 h. The
+  plain FP32 path is held to the same bar, and FAST_MATH must not be far worse
+  than plain FP32.
 * lf_diagnostics: FP64 mass / momentum / density error against numpy FP64
   sums of the downloaded state, and reproducible call to call.
 * The density-error convergence check: a tolerance that every scene meets
@@ -24,7 +27,12 @@ pytestmark = pytest.mark.gpu
 
 DENSITY_REL_TOL = 1e-4
 LAMBDA_TOL = 1e-4      # of max |lambda|
+LAMBDA_TOL_PACKED = 5e-4
 POSITION_TOL = 1e-4    # of h (solver units)
+
+
+def _lambda_tol(flags):
+    return LAMBDA_TOL_PACKED if flags & lf.LF_FLAG_PACKED16 else LAMBDA_TOL
 
 
 def _by_id(values, order):
@@ -85,14 +93,15 @@ def ref64():
 
 
 @pytest.mark.parametrize("flags", [0, lf.LF_FLAG_KAHAN, lf.LF_FLAG_FAST_MATH,
-                                   lf.LF_FLAG_FAST_MATH | lf.LF_FLAG_KAHAN])
+                                   lf.LF_FLAG_FAST_MATH | lf.LF_FLAG_KAHAN, lf.LF_FLAG_PACKED16,
+                                   lf.LF_FLAG_PACKED16 | lf.LF_FLAG_FAST_MATH | lf.LF_FLAG_KAHAN])
 def test_one_step_error_vs_fp64(ref64, flags):
     sc, ref = ref64
     got = _gpu_one_step(sc, flags | lf.LF_FLAG_DIFFUSE)
     e_rho, e_lam, e_pos = _errors(got, ref, sc)
     print("flags %#x: density %.3e lambda %.3e position %.3e" % (flags, e_rho, e_lam, e_pos))
     assert e_rho <= DENSITY_REL_TOL
-    assert e_lam <= LAMBDA_TOL
+    assert e_lam <= _lambda_tol(flags)
     assert e_pos <= POSITION_TOL
 
 
@@ -188,6 +197,7 @@ def _report(tag, errs):
 
 
 MIXED = lf.LF_FLAG_FAST_MATH | lf.LF_FLAG_KAHAN
+PACKED = MIXED | lf.LF_FLAG_PACKED16
 
 
 def test_error_report_4m_one_step():
@@ -195,10 +205,12 @@ def test_error_report_4m_one_step():
     (teacher forced): plain FP32 and mixed (FAST_MATH + KAHAN) against FP64."""
     sc = scenes.double_dam_break_4m()
     ref = _oracle_one_step(sc, lf.LF_FLAG_DIFFUSE, fp64=True)
-    for tag, flags in (("4M one step, plain FP32", 0), ("4M one step, mixed", MIXED)):
+    for tag, flags in (("4M one step, plain FP32", 0), ("4M one step, mixed", MIXED),
+                       ("4M one step, mixed + packed16", PACKED)):
         errs = _errors(_gpu_one_step(sc, flags | lf.LF_FLAG_DIFFUSE), ref, sc)
         _report(tag, errs)
-        assert errs[0] <= DENSITY_REL_TOL and errs[1] <= LAMBDA_TOL and errs[2] <= POSITION_TOL
+        assert errs[0] <= DENSITY_REL_TOL and errs[1] <= _lambda_tol(flags)
+        assert errs[2] <= POSITION_TOL
 
 
 def test_error_report_accumulated_20_steps():
@@ -207,9 +219,23 @@ def test_error_report_accumulated_20_steps():
     sc = scenes.dam_break_8k()
     steps = 20
     ref = _oracle_state(sc, lf.LF_FLAG_DIFFUSE, True, steps)
-    for tag, flags in (("8k 20 steps, plain FP32", 0), ("8k 20 steps, mixed", MIXED)):
+    for tag, flags in (("8k 20 steps, plain FP32", 0), ("8k 20 steps, mixed", MIXED),
+                       ("8k 20 steps, mixed + packed16", PACKED)):
         got = _gpu_state(sc, flags | lf.LF_FLAG_DIFFUSE, steps)
         errs = _errors(got, ref, sc)
         _report(tag, errs)
         # accumulated: a looser bar than one step (errors grow with the steps)
         assert errs[0] <= 1e-3 and errs[2] <= 1e-3
+
+
+def test_packed16_needs_a_packed_grid():
+    sc = scenes.small_block(n_side=6)
+    with lf.Context(0) as c:
+        c.upload(sc.position, sc.velocity, sc.mass)
+        c.grid_build(sc.params())
+        with pytest.raises(lf.LfError):
+            c.lambda_pass(sc.params(flags=lf.LF_FLAG_PACKED16))
+        c.grid_build(sc.params(flags=lf.LF_FLAG_PACKED16))
+        c.lambda_pass(sc.params(flags=lf.LF_FLAG_PACKED16))
+        c.delta_pass(sc.params(flags=lf.LF_FLAG_PACKED16))
+        c.sync()
