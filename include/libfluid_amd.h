@@ -289,6 +289,22 @@ int lf_mc_download(lf_ctx* ctx, lf_mesh_triangle* out, size_t cap,
 /* Maximum triangles kept on the device (default 16M); LF_ERR_CAPACITY is
  * reported by lf_mc_count / lf_mc_download when an emit overflowed it. */
 int lf_mc_set_capacity(lf_ctx* ctx, size_t max_triangles);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Marching cubes of a scalar field already in device
+ * memory, with nothing allocated: d_field on a samples[0] x samples[1] x
+ * samples[2] lattice (x-major, sample (i,j,k) at (origin + (i,j,k) * step) *
+ * scale), the vertex positions of the first tri_cap triangles into d_tri_pos
+ * (9 floats per triangle, cube-index order; the values lf_mc_field_upload +
+ * lf_mc_classify_count + lf_mc_emit give), the full triangle count into
+ * *d_ntri (device memory; above tri_cap means the output was cut). Enqueued
+ * on the context's stream (lf_sync waits). The per-block counts live in a
+ * fixed 1 MiB device array per GPU that every call on that GPU shares: calls
+ * that overlap in time on one GPU (two contexts, two host threads) race. At
+ * most 2^28 cubes ((S0-1)(S1-1)(S2-1)); larger lattices get LF_ERR_ARG. */
+int lf_mc_mesh_static(lf_ctx* ctx, const uint32_t samples[3], const float origin[3],
+                      const float step[3], float scale, float isolevel, const float* d_field,
+                      float* d_tri_pos, size_t tri_cap, uint32_t* d_ntri);
 
 /* ---- the drop-in step ----------------------------------------------------- */
 /* The core of ocl::SphSolver::advance (oclsph.hpp:661-936) after sources and

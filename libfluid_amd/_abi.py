@@ -221,6 +221,7 @@ SIGNATURES = {
     "lf_mc_count": (_I, [_P, ctypes.POINTER(_SZ)]),
     "lf_mc_download": (_I, [_P, _P, _SZ, ctypes.POINTER(_SZ)]),
     "lf_mc_set_capacity": (_I, [_P, _SZ]),
+    "lf_mc_mesh_static": (_I, [_P, _P, _P, _P, ctypes.c_float, ctypes.c_float, _P, _P, _SZ, _P]),
     "lf_advance": (_I, [_P, ctypes.POINTER(PbfParams), ctypes.POINTER(McParams), _P, _SZ, _P,
                         _SZ, ctypes.POINTER(_SZ), _P]),
     "lf_mc_field_upload": (_I, [_P, _P, _P, _P, ctypes.c_float, _P]),

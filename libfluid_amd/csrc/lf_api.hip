@@ -400,6 +400,12 @@ int lf_ctx_create(int device, lf_ctx** out) {
     delete c;
     return LF_ERR_HIP;
   }
+  // the address of lf_mc_mesh_static's block-count array on this device
+  if (int rc = lf::mcStaticInit(device, &c->mcScratch)) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
 
   *out = c;
   return LF_OK;
@@ -834,6 +840,17 @@ int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[
   c->mcSampled = true;
   c->mcCounted = c->mcEmitted = false;
   return LF_OK;
+}
+
+int lf_mc_mesh_static(lf_ctx* c, const uint32_t samples[3], const float origin[3],
+                      const float step[3], float scale, float isolevel, const float* d_field,
+                      float* d_tri_pos, size_t tri_cap, uint32_t* d_ntri) {
+  if (!c || !samples || !origin || !step || !d_field || !d_ntri || (tri_cap && !d_tri_pos))
+    return LF_ERR_ARG;
+  if (!(scale > 0.0f)) return fail(c, LF_ERR_ARG, "lf_mc_mesh_static: scale must be > 0");
+  LF_HIP(c, hipSetDevice(c->device));
+  return lf::mcMeshStatic(c, samples, origin, step, scale, isolevel, d_field, d_tri_pos, tri_cap,
+                          d_ntri);
 }
 
 int lf_mc_set_capacity(lf_ctx* c, size_t max_triangles) {

@@ -648,6 +648,7 @@ struct Ctx {
   bool mcSampled = false, mcCounted = false, mcEmitted = false;
   int mcAttr = 0;
   bool keepCases = false;
+  uint32_t* mcScratch = nullptr;  // lf_mc_mesh_static's block counts on this device
   float mcIso = 0.0f;  // isolevel of the last classify pass (its cases feed the emit)
 
   // profiling
@@ -815,6 +816,10 @@ int pbfDiffuse(Ctx* c);
 int mcSample(Ctx* c, const lf_mc_params& m);   // lf_mc.hip
 int mcClassifyCount(Ctx* c, const lf_mc_params& m);
 int mcEmit(Ctx* c, const lf_mc_params& m);
+int mcStaticInit(int device, uint32_t** out);  // device current
+int mcMeshStatic(Ctx* c, const uint32_t S[3], const float origin[3], const float step[3],
+                 float scale, float iso, const float* field, float* triPos, size_t triCap,
+                 uint32_t* ntri);
 int copyProbe(Ctx* c, size_t bytes, int reps, float* ms);  // lf_grid.hip
 int applyDrains(Ctx* c);                      // lf_grid.hip: stable erase, updates n
 int answerQueries(Ctx* c);

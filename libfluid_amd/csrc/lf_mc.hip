@@ -6,6 +6,8 @@
 // Lattice layout: sample (x,y,z) at index (x*Sy + y)*Sz + z (index3d,
 // curves.h:17-20); cube (x,y,z) at (x*My + y)*Mz + z with M = S - 1 (to3d,
 // curves.h:22-27). Corner order: CUBE_OFFSETS of oclsph_kernel.h:533-542.
+#include <mutex>
+
 #include "fluid/mctable.hpp"
 #include "lf_internal.hpp"
 
@@ -410,7 +412,8 @@ __device__ inline uint32_t cubeCase(const CubeGrid& g, const float* __restrict__
 }
 
 // mc_size (oclsph_kernel.h:545-596): triangles per block of kCubesPerBlock
-// cubes. Cube m = block*1024 + j*256 + thread (j = 0..3).
+// cubes. Cube m = block*1024 + j*256 + thread (j = 0..3). The case indices
+// are kept (one byte per cube) when cases is not null.
 __global__ __launch_bounds__(kThreads) void k_mc_count(CubeGrid g, const float* __restrict__ field,
                                                        float iso, uint32_t* __restrict__ blockCount,
                                                        uint8_t* __restrict__ cases) {
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(kThreads) void k_mc_count(CubeGrid g, const float* 
     if (j > 0) q.advance(g, kThreads);
     if (m < g.cubes) {
       const uint32_t ci = cubeCase(g, field, q, iso);
-      cases[m] = uint8_t(ci);
+      if (cases) cases[m] = uint8_t(ci);
       sum += cTab.ntri[ci];
     }
   }
@@ -541,11 +544,13 @@ __device__ inline EdgeVertex edgeVertex(const EmitArgs& A, const float* __restri
 // interpolation t = (iso - v0)/(v1 - v0), mix(p0, p1, t); triangles of
 // TriTable[ci] written at the scanned offset, in cube order. Blocks without
 // triangles exit at once. The case indices come from the classify pass (one
-// byte per cube). Each cube with triangles stamps its block-local index over
+// byte per cube), or, with kCases false, are classified again from the field
+// (no case array). Each cube with triangles stamps its block-local index over
 // its slots in LDS, then thread k emits the block's triangle k: the threads
 // share the work evenly and consecutive threads write consecutive records.
 constexpr uint32_t kMaxBlockTriangles = kCubesPerBlock * 5;  // at most 5 per cube
 
+template <bool kCases>
 __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* __restrict__ field,
                                                       const uint8_t* __restrict__ cases,
                                                       const float4* __restrict__ normals,
@@ -568,7 +573,14 @@ __global__ __launch_bounds__(kThreads) void k_mc_emit(EmitArgs A, const float* _
   for (uint32_t j = 0; j < uint32_t(kCubesPerThread); ++j) {
     const uint32_t ib = j * kThreads + threadIdx.x;  // block-local cube, cube order
     const uint32_t m = base + ib;
-    const uint32_t ci = m < A.g.cubes ? uint32_t(cases[m]) : 0u;
+    uint32_t ci = 0;
+    if (m < A.g.cubes) {
+      if constexpr (kCases) {
+        ci = cases[m];
+      } else {
+        ci = cubeCase(A.g, field, CubeCursor(A.g, m), A.iso);
+      }
+    }
     caseOf[ib] = uint8_t(ci);
     const uint32_t nt = cTab.ntri[ci];
     uint32_t tot;
@@ -837,7 +849,7 @@ int mcEmit(Ctx* c, const lf_mc_params& m) {
   A.cap = c->triCap;
   A.attributes = attr;
   if (nbl > 0) {
-    rc = launch(c, "mc_emit", k_mc_emit, dim3(unsigned(nbl)), dim3(kThreads), 0, A,
+    rc = launch(c, "mc_emit", k_mc_emit<true>, dim3(unsigned(nbl)), dim3(kThreads), 0, A,
                 (const float*)c->field.p, (const uint8_t*)c->mcCases.p,
                 (const float4*)((attr && c->mcAttr) ? c->mcNormal.p : nullptr),
                 (const uint32_t*)((attr && c->mcAttr) ? c->mcColour.p : nullptr),
@@ -847,6 +859,78 @@ int mcEmit(Ctx* c, const lf_mc_params& m) {
     if (rc) return rc;
   }
   c->mcEmitted = true;
+  return LF_OK;
+}
+
+// ---- meshing without allocation (lf_mc_mesh_static) ---------------------------
+
+// The per-block triangle counts (then offsets) of lf_mc_mesh_static: a
+// file-scope device array, so the call allocates nothing. Each GPU holds its
+// own copy until the process ends; every call on that GPU shares it, and
+// kernel arguments take the address hipGetSymbolAddress returned on that
+// device (the array's name is a host-side stand-in). The count pass writes
+// every entry it uses, so nothing has to start at zero.
+constexpr uint32_t kStaticMaxBlocks = 1u << 18;  // 2^28 cubes, 1 MiB
+static __device__ __align__(256) uint32_t scratch[kStaticMaxBlocks];
+
+constexpr int kMaxDevices = 64;
+static std::mutex gScratchMutex;
+static uint32_t* gScratch[kMaxDevices];  // the array's address on each device
+
+int mcStaticInit(int device, uint32_t** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return LF_ERR_ARG;
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (!gScratch[device]) {
+    void* p = nullptr;  // device is current
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LF_ERR_HIP;
+    gScratch[device] = static_cast<uint32_t*>(p);
+  }
+  *out = gScratch[device];
+  return LF_OK;
+}
+
+int mcMeshStatic(Ctx* c, const uint32_t S[3], const float origin[3], const float step[3],
+                 float scale, float iso, const float* field, float* triPos, size_t triCap,
+                 uint32_t* ntri) {
+  if (!c->mcScratch) return fail(c, LF_ERR_STATE, "lf_mc_mesh_static: no block-count array on this device");
+  uint64_t cubes = 1;
+  CubeGrid g{};
+  for (int i = 0; i < 3; ++i) {
+    if (S[i] < 2) return fail(c, LF_ERR_ARG, "lf_mc_mesh_static: samples must be >= 2 per axis");
+    g.S[i] = S[i];
+    g.M[i] = S[i] - 1u;
+    cubes *= g.M[i];
+  }
+  const uint64_t nbl = (cubes + kCubesPerBlock - 1) / kCubesPerBlock;
+  if (nbl > kStaticMaxBlocks)
+    return fail(c, LF_ERR_ARG, "lf_mc_mesh_static: " + std::to_string(cubes) +
+                                   " cubes; at most " +
+                                   std::to_string(uint64_t(kStaticMaxBlocks) * kCubesPerBlock));
+  g.cubes = uint32_t(cubes);
+  EmitArgs A{};
+  A.g = g;
+  A.iso = iso;
+  for (int i = 0; i < 3; ++i) {
+    A.origin[i] = origin[i];
+    A.step[i] = step[i];
+  }
+  A.scale = scale;
+  A.cap = triCap;
+  A.attributes = 0;
+  // plain launches on the context's stream: no timing events, no buffers
+  hipLaunchKernelGGL(k_mc_count, dim3(unsigned(nbl)), dim3(kThreads), 0, c->stream, g, field, iso,
+                     c->mcScratch, (uint8_t*)nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hipFail(c, e, "mc_classify_count");
+  hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, c->stream, c->mcScratch, uint32_t(nbl), ntri);
+  if ((e = hipGetLastError()) != hipSuccess) return hipFail(c, e, "mc_scan");
+  if (triCap == 0) return LF_OK;
+  hipLaunchKernelGGL(k_mc_emit<false>, dim3(unsigned(nbl)), dim3(kThreads), 0, c->stream, A, field,
+                     (const uint8_t*)nullptr, (const float4*)nullptr, (const uint32_t*)nullptr,
+                     (const uint32_t*)c->mcScratch, (const uint32_t*)ntri, uint32_t(nbl), triPos,
+                     (float*)nullptr, (uint32_t*)nullptr);
+  if ((e = hipGetLastError()) != hipSuccess) return hipFail(c, e, "mc_emit");
   return LF_OK;
 }
 
