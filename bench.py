@@ -81,10 +81,13 @@ def grid_build_bytes(n, G):
 # a pattern of the full name where template arguments tell launches apart.
 KERNEL_SYMBOL = {
     "predict_count": "k_predict_count", "bucket_reduce": "k_bucket_reduce",
-"bucket_scan": "k_bucket_scan",
-    "bucket_scatter": "k_bucket_scatter", "sort_big": "k_sort_big", "reorder": "k_reorder",
-    "pbf_build_list": "k_build_list", "pbf_lambda": "k_lambda", "pbf_lambda_kahan": "k_lambda",
-    "pbf_delta": r"k_delta<\d+, false, false>", "pbf_delta_final": r"k_delta<\d+, false, true>",
+    "bucket_scan": "k_bucket_scan", "bucket_scatter": "k_bucket_scatter",
+    "sort_big": "k_sort_big", "reorder": "k_reorder", "pbf_build_list": "k_build_list",
+    # k_lambda<threads, KAHAN, FAST_MATH, PACKED16>, k_delta<threads, FAST_MATH, FINAL, PACKED16>
+    "pbf_lambda": r"k_lambda<\d+, false, false, false>",
+    "pbf_lambda_kahan": r"k_lambda<\d+, true, false, false>",
+    "pbf_delta": r"k_delta<\d+, false, false, false>",
+    "pbf_delta_final": r"k_delta<\d+, false, true, false>",
     "pbf_finalise": "k_finalise", "pbf_xsph": "k_xsph",
     "pbf_diffuse": "k_diffuse", "mc_density_field": "k_mc_density", "mc_classify_count": "k_mc_count",
     "mc_scan": "k_mc_scan", "mc_emit": "k_mc_emit", "copy_probe": "k_copy_probe",
@@ -170,9 +173,10 @@ def summarize_trace(path, steps, warmup, iterations=4):
     scene, the steps bench.py's value and roofline are measured on). Steps are
     counted by the launches of k_predict_count (one per step)."""
     rows = load_trace(path)
-    sym = lambda name: re.sub(r"^void ", "", re.sub(r"\(.*", "", name))  # noqa: E731
-    starts = [r[1] for r in rows if sym(r[0]).startswith("lf::(anonymous namespace)::k_predict_count")
-              or sym(r[0]).startswith("lf::k_predict_count")]
+    # "void lf::(anonymous namespace)::k_delta<256, ...>(lf::Work, ...)" -> "lf::k_delta<256, ...>"
+    sym = lambda name: re.sub(r"\(.*", "", re.sub(r"^void ", "", name.replace(  # noqa: E731
+        "(anonymous namespace)::", "")))
+    starts = [r[1] for r in rows if sym(r[0]).startswith("lf::k_predict_count")]
     per_pass = warmup + steps
     t0 = starts[per_pass + warmup] if len(starts) > per_pass + warmup else None
     t1 = starts[2 * per_pass] if len(starts) > 2 * per_pass else (rows[-1][2] + 1)
@@ -180,7 +184,7 @@ def summarize_trace(path, steps, warmup, iterations=4):
     def stats(sel):
         acc = {}
         for name, a, b in sel:
-            k = sym(name).replace("lf::(anonymous namespace)::", "").replace("lf::", "")
+            k = sym(name).replace("lf::", "")
             acc.setdefault(k, []).append((b - a) / 1000.0)
         return acc
 

@@ -1,5 +1,6 @@
-// Neighbour-search grid on gfx950: predict + Morton key, stable radix sort
-// (lf_sort.hip), payload reorder and the dense cell-start table.
+// Neighbour-search grid on gfx950: predict + Morton key, and the stable sort
+// and the dense cell-start table as one counting sort over the table's cells,
+// with the payload reorder.
 //
 // Replaces the host pre-pass of ocl::SphSolver::advance: advectAndCopy
 // (oclsph.hpp:289-325), computeBoundAndZindex (:327-387), the sort (:733-743)
