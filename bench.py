@@ -357,6 +357,8 @@ def main():
     ap.add_argument("--skin", type=float, default=None,
                     help="neighbour-list skin (fraction of h) of the 1M runs (default: the "
                          "library's, 0.2)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="work-queue launches for the pair kernels of the 1M runs")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the per-kernel HBM traffic table of two rocprofv3 --pmc runs")
     ap.add_argument("--summarize-trace", metavar="RESULTS",
@@ -387,8 +389,8 @@ def main():
           file=sys.stderr, flush=True)
 
     ctx = lf.Context(0)
-    if args.skin is not None:
-        ctx.exec_config(256, False, args.skin)
+    if args.skin is not None or args.persistent:
+        ctx.exec_config(256, args.persistent, 0.2 if args.skin is None else args.skin)
     ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
 
     def one_step():

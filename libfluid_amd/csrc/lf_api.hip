@@ -268,9 +268,12 @@ int checkHealth(Ctx* c, bool wait) {
 }
 
 // Clears the health counters (a new state), after the steps in flight (they
-// write the host copy).
+// write the host copy), and the per-cell list displacements (the next grid
+// build clears only the cells of the particles it is given).
 static int resetHealth(Ctx* c) {
   if (c->health.p) LF_HIP(c, hipMemsetAsync(c->health.p, 0, 4 * sizeof(uint32_t), c->stream));
+  if (c->cellDisp.p)
+    LF_HIP(c, hipMemsetAsync(c->cellDisp.p, 0, c->cellDisp.cap * sizeof(uint32_t), c->stream));
   LF_HIP(c, hipStreamSynchronize(c->stream));
   if (c->healthHost) std::memset(c->healthHost, 0, 4 * sizeof(uint32_t));
   return LF_OK;

@@ -24,9 +24,8 @@ constexpr int kThreads = 256;
 //   B k_bucket_reduce  the sum of each tile of kBucketTile bucket counts
 //     k_bucket_scan    exclusive scan of the counts = the table (first sorted
 //                      index of every bucket, oclsph.hpp:758-765), each tile
-//                      after the sum of the earlier tiles' sums; clears the
-//                      counts and the per-cell list displacements
-//   C k_bucket_scatter slot = table[key] + provisional rank
+//                      after the sum of the earlier tiles' sums
+//   C k_bucket_scatter slot = table[key] + provisional rank; clears the counts
 //   E k_sort_big       buckets of more than kFixupMax particles: their
 //                      indices sorted within the bucket (one workgroup each)
 //   D k_reorder        final slot = table[key] + the number of the bucket's
@@ -53,18 +52,31 @@ __global__ __launch_bounds__(BS) void k_predict_count(Work W, const float4* __re
                                                       int nWells, uint32_t* __restrict__ count,
                                                       uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ ranks,
-                                                      uint32_t* __restrict__ health) {
+                                                      uint32_t* __restrict__ health,
+                                                      const uint32_t* __restrict__ prevKey,
+                                                      uint32_t* __restrict__ cellDisp) {
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
     const uint32_t i = ch.begin + threadIdx.x;
     const bool valid = i < ch.end;
-    uint32_t key = 0xFFFFFFFFu;
+    uint32_t key = 0xFFFFFFFFu, old = 0xFFFFFFFFu;
     if (valid) {
       float px, py, pz;
       predictPstar(c, X[i], V[i], T[i], wells, nWells, px, py, pz);
       if (!(isfinite(px) && isfinite(py) && isfinite(pz))) atomicAdd(&health[0], 1u);
       key = c.keyBase + min(cellKey(c, px, py, pz), c.G);
+      old = prevKey[i];
     }
     const uint32_t lane = __lane_id();
+    // The cells whose displacement the last step's delta passes raised are
+    // the cells of its sorted particles, which are this step's input in the
+    // same order: each run of them is cleared once, before this step's passes
+    // raise them again. (Input that did not come from the last step, after
+    // an upload or a drain, leaves some cells at old values: larger than
+    // this step's, so the list tests only become stricter.) The keys are
+    // bounded without wrapping: they may be any value on the first step.
+    const uint32_t prevOld = __shfl_up(old, 1, 64);
+    if (valid && (lane == 0 || prevOld != old) && old < c.G - 1u && c.G > 0u)
+      cellDisp[c.keyBase + old] = 0u;
     const uint32_t prev = __shfl_up(key, 1, 64);
     const bool start = valid && (lane == 0 || prev != key);
     const uint64_t starts = __ballot(start);
@@ -117,14 +129,11 @@ __global__ __launch_bounds__(kThreads) void k_bucket_reduce(const uint32_t* __re
 // B: the table from the counts. A workgroup's offset is the sum of the
 // earlier tiles' sums (each workgroup adds them up itself: a few KB of reads,
 // no separate scan launch); thread j then scans buckets [16j, 16j + 16) of the
-// tile after the sums of the workgroup's earlier threads. Clears the counts
-// for the next step and the per-cell displacements of this step's lists;
-// table[nb] = n.
-__global__ __launch_bounds__(kThreads) void k_bucket_scan(uint32_t* __restrict__ count,
+// tile after the sums of the workgroup's earlier threads; table[nb] = n.
+__global__ __launch_bounds__(kThreads) void k_bucket_scan(const uint32_t* __restrict__ count,
                                                           uint32_t nb,
                                                           const uint32_t* __restrict__ tileSum,
-                                                          uint32_t* __restrict__ table,
-                                                          uint32_t* __restrict__ cellDisp) {
+                                                          uint32_t* __restrict__ table) {
   __shared__ uint32_t sm[kThreads / 64];
   __shared__ uint32_t tileOff;
   const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
@@ -184,32 +193,24 @@ __global__ __launch_bounds__(kThreads) void k_bucket_scan(uint32_t* __restrict__
   }
   if (whole) {
     uint4* qt = reinterpret_cast<uint4*>(table + b0);
-    uint4* qc = reinterpret_cast<uint4*>(count + b0);
-    uint4* qd = reinterpret_cast<uint4*>(cellDisp + b0);
-    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-    for (uint32_t j = 0; j < kBucketItems / 4; ++j) {
+    for (uint32_t j = 0; j < kBucketItems / 4; ++j)
       qt[j] = make_uint4(t[4 * j], t[4 * j + 1], t[4 * j + 2], t[4 * j + 3]);
-      qc[j] = z;
-      qd[j] = z;
-    }
   } else {
 #pragma unroll
     for (uint32_t j = 0; j < kBucketItems; ++j)
-      if (b0 + j < nb) {
-        table[b0 + j] = t[j];
-        count[b0 + j] = 0u;
-        cellDisp[b0 + j] = 0u;
-      }
+      if (b0 + j < nb) table[b0 + j] = t[j];
   }
 }
 
 // C: each particle to its provisional slot. The first particle of a bucket
-// (provisional rank 0) lists the bucket for E when it holds more than
-// kFixupMax particles, and counts it as overfull when it is a cell of the
-// table holding more than `guard` particles (guard 0: no limit).
+// (provisional rank 0) clears the bucket's count for the next step (only
+// buckets holding particles were counted), lists the bucket for E when it
+// holds more than kFixupMax particles, and counts it as overfull when it is
+// a cell of the table holding more than `guard` particles (guard 0: no limit).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_bucket_scatter(Work W, uint32_t n,
+                                                       uint32_t* __restrict__ count,
                                                        const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ ranks,
                                                        const uint32_t* __restrict__ table,
@@ -228,6 +229,7 @@ __global__ __launch_bounds__(BS) void k_bucket_scatter(Work W, uint32_t n,
       tmpVal[pos] = i;
     }
     if (r == 0u) {
+      count[k] = 0u;
       const uint32_t len = table[k + 1u] - s;
       if (len > kFixupMax) {
         const uint32_t j = atomicAdd(&big[0], 1u);  // below n / (kFixupMax + 1) < bigCap
@@ -248,7 +250,7 @@ __device__ inline uint64_t laneMaskLt() { return (uint64_t(1) << __lane_id()) - 
 // `passes` passes (even, so the result ends where it started), ping-ponging
 // with the same range of `scratch`. Indices are distinct, so the order is the
 // stable one. The last workgroup to leave clears the list for the next step.
-constexpr int kBigThreads = 1024;
+constexpr int kBigThreads = 256;  // small, so that the (usually empty) launch finds room beside other work
 constexpr int kBigWaves = kBigThreads / 64;
 __global__ __launch_bounds__(kBigThreads) void k_sort_big(uint32_t* __restrict__ big,
                                                           const uint32_t* __restrict__ table,
@@ -679,7 +681,7 @@ int gridBuild(Ctx* c) {
   if ((rc = ensure(c, c->P, n, "p*"))) return rc;
   if ((rc = ensure(c, c->Palt, n, "p*"))) return rc;  // both pairs: no growth in a graph
   if ((rc = ensure(c, c->PL, n, "p*+lambda"))) return rc;
-  if ((rc = ensure(c, c->localKey, n, "cell keys"))) return rc;
+  if ((rc = ensureZeroed(c, c->localKey, n, "cell keys"))) return rc;
   if ((rc = ensure(c, c->nbr, nbrCapacity(n), "neighbour lists"))) return rc;
   if ((rc = ensure(c, c->nbrCount, n, "neighbour counts"))) return rc;
   if ((rc = ensure(c, c->P0, n, "list positions"))) return rc;
@@ -690,7 +692,7 @@ int gridBuild(Ctx* c) {
   if ((rc = ensure(c, c->diffused, n, "diffused"))) return rc;
   if ((rc = ensure(c, c->table, size_t(nb) + 1, "cell table"))) return rc;
   if ((rc = ensure(c, c->tableAlt, size_t(nb) + 1, "cell table"))) return rc;
-  if ((rc = ensure(c, c->cellDisp, size_t(nb), "cell displacement"))) return rc;
+  if ((rc = ensureZeroed(c, c->cellDisp, size_t(nb), "cell displacement"))) return rc;
   if ((rc = ensureZeroed(c, c->cellCount, size_t(nb), "bucket counts"))) return rc;
   if ((rc = ensure(c, c->bucketTile, size_t(nTiles), "bucket tiles"))) return rc;
   if ((rc = ensureHealthHost(c))) return rc;
@@ -708,15 +710,16 @@ int gridBuild(Ctx* c) {
   rc = LF_DISPATCH_BS(c, "predict_count", k_predict_count, W, (const float4*)c->X[c->cur].p,
                       (const float4*)c->V[c->cur].p, (const uint8_t*)c->T[c->cur].p,
                       (const lf_well*)c->wells.p, c->nWells, c->cellCount.p, c->sortKey[1].p,
-                      c->sortVal[1].p, c->health.p);
+                      c->sortVal[1].p, c->health.p, (const uint32_t*)c->localKey.p,
+                      c->cellDisp.p);
   if (rc) return rc;
   rc = launch(c, "bucket_reduce", k_bucket_reduce, dim3(nTiles), dim3(kThreads), 0,
               (const uint32_t*)c->cellCount.p, nb, c->bucketTile.p);
   if (rc) return rc;
-  rc = launch(c, "bucket_scan", k_bucket_scan, dim3(nTiles), dim3(kThreads), 0, c->cellCount.p, nb,
-              (const uint32_t*)c->bucketTile.p, c->table.p, c->cellDisp.p);
+  rc = launch(c, "bucket_scan", k_bucket_scan, dim3(nTiles), dim3(kThreads), 0,
+              (const uint32_t*)c->cellCount.p, nb, (const uint32_t*)c->bucketTile.p, c->table.p);
   if (rc) return rc;
-  rc = LF_DISPATCH_BS(c, "bucket_scatter", k_bucket_scatter, W, uint32_t(n),
+  rc = LF_DISPATCH_BS(c, "bucket_scatter", k_bucket_scatter, W, uint32_t(n), c->cellCount.p,
                       (const uint32_t*)c->sortKey[1].p, (const uint32_t*)c->sortVal[1].p,
                       (const uint32_t*)c->table.p, c->sortKey[0].p, c->sortVal[0].p,
                       c->bigBuckets.p, uint32_t(c->bigBuckets.cap - kBigHeader), c->guardCell,

@@ -119,11 +119,27 @@ struct Work {
 
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD
 // b % 8), each XCD with its own L2. This maps them so that XCD x works through
-// one contiguous eighth of the chunks: neighbouring particles, whose data the
-// pair passes gather, then share an L2. A bijection on [0, n).
+// stripes of LF_XCD_STRIPE consecutive chunks (4,096 particles at 256 per
+// chunk), stripe j on XCD j % 8: neighbouring particles, whose data the pair
+// passes gather, mostly share an L2, and a region of costly chunks (a splash
+// whose lists are rebuilt) is spread over every XCD. Contiguous eighths per
+// XCD (LF_XCD_STRIPE 0) left that work to one XCD: 1.936 against 1.783
+// ms/step over steps 5-25 of the 1M dam break (stripes of 4: 1.782, of 1, the
+// identity: 1.794). A bijection on [0, n).
+#ifndef LF_XCD_STRIPE
+#define LF_XCD_STRIPE 16
+#endif
 __device__ inline uint32_t xcdChunk(uint32_t b, uint32_t n) {
-  const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, i = b >> 3;
-  return x * q + min(x, r) + i;
+  const uint32_t x = b & 7u, i = b >> 3;
+  if constexpr (LF_XCD_STRIPE == 0) {
+    const uint32_t q = n >> 3, r = n & 7u;
+    return x * q + min(x, r) + i;
+  } else {
+    // stripes of S chunks dealt to the XCDs in turn (S = 1: the identity)
+    constexpr uint32_t S = LF_XCD_STRIPE;
+    if (b >= (n / (8u * S)) * (8u * S)) return b;
+    return (i / S) * (8u * S) + x * S + i % S;
+  }
 }
 
 // Runs body(consts, chunk) for the chunks of this workgroup: one chunk per

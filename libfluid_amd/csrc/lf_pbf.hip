@@ -662,7 +662,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
       const uint32_t otherKey = __shfl_down(key, off, 64);
       if (__lane_id() + off < 64 && otherKey == key) runMax = fmaxf(runMax, other);
     }
-    if (valid && head && runMax > 0.0f && key + 1u < c.G)
+    if (valid && head && runMax > 0.0f && key < c.G - 1u && c.G > 0u)
       atomicMax(&L.cellDisp[c.keyBase + key], __float_as_uint(runMax));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
