@@ -49,6 +49,10 @@ constexpr int kBlockXsph = LF_BLOCK_XSPH;
 #define LF_BLOCK_BUILD 4
 #endif
 constexpr int kBuildBlock = LF_BLOCK_BUILD;  // candidates loaded together (list build)
+#ifndef LF_REBUILD_BLOCK
+#define LF_REBUILD_BLOCK 4
+#endif
+constexpr int kRebuildBlock = LF_REBUILD_BLOCK;  // the same, lists rebuilt by a lambda pass
 // lists are padded (with the particle's own index) to whole blocks of this
 // many entries, the largest block a walk reads
 constexpr uint32_t kListPad = 16;
@@ -154,14 +158,14 @@ __device__ inline const float4& posOf(const PosQ& r) { return r.p; }
 // that rebuilds lists evaluates its pair terms in the same walk. Each lane
 // collects its accepted candidates in a 16-entry LDS ring (win, two 8-entry
 // groups) and, once per block of candidates, stores a completed group as 24
-// bytes of its list: a block adds at most kBuildBlock <= 8 entries, so at most
+// bytes of its list: a block adds at most BB <= 8 entries, so at most
 // one group completes in it and the group still to be stored is never
 // overwritten. (Checking after every candidate ran the store path on almost
 // every step: with 64 lanes, some lane nearly always completes a group.) The
 // list is padded with a itself to whole blocks of kListPad entries; the
 // count (kMaxNbr + 1: overflowed, or a candidate 2^23 positions away) goes
 // to L.count[a].
-template <int BS, typename Load, typename Visit>
+template <int BS, int BB = kBuildBlock, typename Load, typename Visit>
 __device__ inline void buildList(const PbfConsts& c, const Lists& L, uint32_t* __restrict__ win,
                                  uint32_t a, float4 pa, uint32_t key,
                                  const uint32_t* __restrict__ table, int checkFar, Load&& load,
@@ -169,7 +173,7 @@ __device__ inline void buildList(const PbfConsts& c, const Lists& L, uint32_t* _
   constexpr uint32_t kRing = 2 * kNbrGroup;
   // at most 7 entries wait before a block, so at most 15 after it: one group
   // completes at most, and a block's writes always find a free slot
-  static_assert(kBuildBlock <= int(kNbrGroup), "a block completes at most one group");
+  static_assert(BB <= int(kNbrGroup), "a block completes at most one group");
   uint32_t cnt = 0, stored = 0;  // entries accepted / written to the list (a multiple of 8)
   bool far = false;  // a candidate 2^23 or more sorted positions away: no list
   // group i / 8 of a's list starts at nbrGroupByte(a, 0) + (i / 8) * 1536 bytes
@@ -190,12 +194,12 @@ __device__ inline void buildList(const PbfConsts& c, const Lists& L, uint32_t* _
   forEachNeighbourCellRolled(key, table, c.G, [&](uint32_t s, uint32_t e) {
     // a cell's candidates are the run [s, e): its ends bound their distance
     if (checkFar && ((s - a + kNbrBias) >> 24 || (e - 1u - a + kNbrBias) >> 24)) far = true;
-    for (uint32_t b0 = s; b0 < e; b0 += kBuildBlock) {
-      decltype(load(0u)) rec[kBuildBlock];  // candidates loaded together (indices clamped into the cell)
+    for (uint32_t b0 = s; b0 < e; b0 += BB) {
+      decltype(load(0u)) rec[BB];  // candidates loaded together (indices clamped into the cell)
 #pragma unroll
-      for (int k = 0; k < kBuildBlock; ++k) rec[k] = load(min(b0 + k, e - 1u));
+      for (int k = 0; k < BB; ++k) rec[k] = load(min(b0 + k, e - 1u));
 #pragma unroll
-      for (int k = 0; k < kBuildBlock; ++k) {
+      for (int k = 0; k < BB; ++k) {
         const uint32_t b = b0 + k;
         const float4& pb = posOf(rec[k]);
         // both tests always evaluated (the load index is clamped into the
@@ -397,7 +401,7 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
   if (fell) usable = false;
   if constexpr (REBUILD) {
     if (!usable && win && passIdx < kPassSlots) {
-      buildList<BS>(c, L, win, a, pa, key, table, checkFar, loadPos,
+      buildList<BS, kRebuildBlock>(c, L, win, a, pa, key, table, checkFar, loadPos,
                     [&](uint32_t b, const auto& r, bool valid) {
                       float dx, dy, dz;
                       rel(r, dx, dy, dz);
