@@ -150,6 +150,48 @@ def summarize_pmc(fetch_dir, write_dir, copy_bytes=1 << 28):
         print("| %s | %.0f | %.0f | %d | fetch x%.3f, write x%.3f |" % (k, fb, wb, total, ff, fw))
 
 
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES",
+               "SQ_WAVE_CYCLES", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_VMEM_RD")
+
+
+def summarize_sq(sq_dir, trace_csv, cus=256, simds=4, ghz=2.4):
+    """Per-kernel means per dispatch of a rocprofv3 --pmc run of SQ_COUNTERS,
+    with the kernel's mean duration from a kernel trace of the same command:
+    `VALU time` = SQ_INSTS_VALU x 4 cycles (a wave64 VALU instruction occupies
+    a SIMD16 for 4 cycles) / (cus x simds x ghz), the time the kernel would take
+    if VALU issue were its only limit, and `VALU share` = that time / the mean
+    duration. Prints a markdown table."""
+    import csv
+    import glob
+    import re
+    acc = {}
+    for f in glob.glob(os.path.join(sq_dir, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            m = re.search(r"(k_[a-z_0-9]+(<[^>]*>)?)\(", r["Kernel_Name"])
+            if m:
+                acc.setdefault(m.group(1), {}).setdefault(r["Counter_Name"], []).append(
+                    float(r["Counter_Value"]))
+    dur = {}
+    for name, a, b in load_trace(trace_csv):
+        m = re.search(r"(k_[a-z_0-9]+(<[^>]*>)?)\(", name)
+        if m:
+            dur.setdefault(m.group(1), []).append((b - a) / 1000.0)
+    print("| kernel | waves | VALU / wave | VMEM rd / wave | LDS / wave | LDS bank conflict cycles "
+          "| VALU time us | trace avg us | VALU share |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|---:|")
+    for k in sorted(acc):
+        c = {n: sum(v) / len(v) for n, v in acc[k].items()}
+        waves = c.get("SQ_WAVES", 0.0)
+        per = lambda n: c.get(n, 0.0) / waves if waves else 0.0  # noqa: E731
+        valu_us = c.get("SQ_INSTS_VALU", 0.0) * 4.0 / (cus * simds * ghz * 1e3)
+        d = dur.get(k)
+        avg = sum(d) / len(d) if d else None
+        print("| %s | %.0f | %.0f | %.1f | %.1f | %.0f | %.1f | %s | %s |" % (
+            k, waves, per("SQ_INSTS_VALU"), per("SQ_INSTS_VMEM_RD"), per("SQ_INSTS_LDS"),
+            c.get("SQ_LDS_BANK_CONFLICT", 0.0), valu_us, "%.1f" % avg if avg else "-",
+            "%.0f%%" % (100.0 * valu_us / avg) if avg else "-"))
+
+
 def load_trace(path):
     """(name, start_ns, end_ns) of every kernel dispatch of a rocprofv3
     --kernel-trace run, in start order: the SQLite results database (ROCm 7's
@@ -361,12 +403,18 @@ def main():
                     help="work-queue launches for the pair kernels of the 1M runs")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the per-kernel HBM traffic table of two rocprofv3 --pmc runs")
+    ap.add_argument("--summarize-sq", nargs=2, metavar=("SQ_DIR", "TRACE_CSV"),
+                    help="print the per-kernel SQ counter table of a rocprofv3 --pmc run with "
+                         "the durations of a kernel trace of the same command")
     ap.add_argument("--summarize-trace", metavar="RESULTS",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace run of this "
                          "script (results .db or kernel_trace.csv) over the --steps/--warmup window")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
+        return
+    if args.summarize_sq:
+        summarize_sq(*args.summarize_sq)
         return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace, args.steps, args.warmup)
