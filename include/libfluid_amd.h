@@ -372,8 +372,10 @@ int lf_exec_config(lf_ctx* ctx, uint32_t threads, int persistent, float skin);
 /* The full execution configuration. particles_per_thread (1, 2 or 4): a
  * workgroup of a launch without the work queue takes that many consecutive
  * chunks, each thread one particle of each. mc_tile: the density field's
- * sample tile per workgroup, 0 = 8x8x4 (x, y, z), 1 = 4x8x8, 2 = 8x4x8. None
- * of these change results. */
+ * sample tile per workgroup of 256 threads, each thread a column of samples
+ * along z: 0 = 8x8x16 samples (x, y, z; 4 per column), 1 = 4x16x16 (4 per
+ * column), 2 = 8x8x8 (2 per column), 3 = 8x8x4 (1 per column). None of these
+ * change results. */
 typedef struct lf_exec_params {
   uint32_t threads;
   int32_t persistent;

@@ -1149,7 +1149,8 @@ int lf_exec_configure(lf_ctx* c, const lf_exec_params* e) {
     return fail(c, LF_ERR_ARG, "lf_exec_configure: skin must be in [0, 1] (fraction of h)");
   if (e->particles_per_thread != 1 && e->particles_per_thread != 2 && e->particles_per_thread != 4)
     return fail(c, LF_ERR_ARG, "lf_exec_configure: particles_per_thread must be 1, 2 or 4");
-  if (e->mc_tile > 2) return fail(c, LF_ERR_ARG, "lf_exec_configure: mc_tile must be 0, 1 or 2");
+  if (e->mc_tile >= lf::kMcTiles)
+    return fail(c, LF_ERR_ARG, "lf_exec_configure: mc_tile must be 0, 1, 2 or 3");
   lf::dropGraph(c);
   c->chunkThreads = e->threads;
   c->persistent = e->persistent != 0;
@@ -1271,7 +1272,7 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
   if (rc == LF_OK && c->haveDensity && copyState(false) && (rc = lf::pbfStepOnce(c)) == LF_OK) {
     const lf_mc_params m = c->lastDensity;
     float bestMc = INFINITY;
-    for (uint32_t t = 0; t < 3 && rc == LF_OK; ++t) {
+    for (uint32_t t = 0; t < lf::kMcTiles && rc == LF_OK; ++t) {
       c->mcTile = t;
       if ((rc = lf::mcSample(c, m))) break;  // warm-up
       if ((rc = lf::mcJoin(c))) break;

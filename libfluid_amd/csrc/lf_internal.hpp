@@ -513,6 +513,8 @@ struct PendingTiming {
   hipEvent_t start, stop;
 };
 
+constexpr uint32_t kMcTiles = 4;  // density-field tile shapes (lf_mc.hip)
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -623,7 +625,7 @@ struct Ctx {
   uint32_t nChunks = 0;
   uint32_t chunkThreads = 256;             // workgroup size of the particle kernels
   uint32_t chunkPpt = 1;                   // chunks per workgroup (particles per thread)
-  uint32_t mcTile = 0;                     // density-field tile shape (lf_exec_params)
+  uint32_t mcTile = 0;                     // density-field tile shape (lf_exec_params), < kMcTiles
   lf_mc_params lastDensity{};              // the last density-field lattice sampled (autotuner)
   bool haveDensity = false;
   std::vector<PbfConsts> uploadedConsts;  // what dConsts holds

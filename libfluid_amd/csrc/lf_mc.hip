@@ -51,40 +51,84 @@ struct Lattice {
 // SPH density field rho(x) = sum over fluid b of m_b poly6 (h^2 - d^2)^3 for
 // d^2 = |x - p*_b|^2 <= h^2 (W6 written on d^2, no square root); P.w is the
 // fluid mass (0 for obstacles) after finalise, staged as m_b poly6.
-// Candidates: the 27 cells around the sample's own cell, reference order.
+// Candidates: the 27 cells around the sample's own cell, reference order
+// (z rows outer, then y rows, each row the x-run of three cells).
 //
-// One workgroup per kTX x kTY x kTZ tile of samples: the particles of the
-// tile's halo cells (its cell range +-1 per axis) are staged once in LDS with
-// a local cell table, so the per-sample candidate loop reads LDS; a tile whose
-// halo holds no particle writes zeros and exits. Tiles whose halo does not fit
-// (coarse lattices, crowded cells) read global memory instead, same order.
-// The tile shape is a runtime choice (lf_exec_configure, the autotuner):
-// 8 x 8 x 4 samples (x, y, z) by default, measured fastest of 12 shapes at
-// 256^3 on the 1M dam break (373-391 us against 597-764 us for 4 x 8 x 8);
-// 4 x 8 x 8 and 8 x 4 x 8 are the alternatives.
+// One workgroup per tile of samples: the particles of the tile's halo cells
+// (its cell range +-1 per axis) are staged once in LDS with a local cell
+// table, so the candidate loops read LDS; a tile whose halo holds no particle
+// writes zeros and exits. Tiles whose halo does not fit (coarse lattices,
+// crowded cells) read global memory instead, same order.
+//
+// Each lane owns a column of kNZ samples along z, and each wave a kWX x kWY
+// block of columns over the same z range, so the samples' z cells are the
+// same on every lane of a wave. The lane walks the union of its samples' z
+// rows once: per candidate one LDS read and the x and y part of the distance,
+// u = fma(-dy, dy, fma(-dx, dx, h^2)), shared by the column, then per sample
+// whose 27 cells contain the row t = max(fma(-dz, dz, u), 0) and
+// rho = fma(w_b t, t t, rho). Which samples a row belongs to is the same on
+// every lane, so the loop for each row is chosen once (a uniform branch).
+// Each sample still sees exactly its 27 cells in the reference order.
+// Outside h, and for a NaN or infinite distance, t is 0 before it is used, so
+// the term is exactly 0, as the test d2 <= h^2 would make it.
 #ifndef LF_MC_UNROLL
 #define LF_MC_UNROLL 4
 #endif
-constexpr uint32_t kUnroll = LF_MC_UNROLL;  // staged particles in flight per sample
+constexpr uint32_t kUnroll = LF_MC_UNROLL;  // staged particles in flight per lane
 constexpr int kHaloCells = 512;
-constexpr int kStageCap = 2048;           // staged particles (32 KB)
 
-// One term of the field, w_b (h^2 - d^2)^3 with w_b = m_b poly6 (formed once
-// per particle), added as rho = fma(w_b t, t t, rho), where t = h^2 - d^2 is
-// one FMA chain from h^2, clamped at 0: 10 VALU operations per candidate.
-// Outside h, and for a NaN or infinite distance, t is 0 before it is used, so
-// the term is exactly 0, as the test d2 <= h^2 would make it.
-struct DensityTerm {
-  float wt, tt;  // w_b t and t t
-};
-__device__ inline DensityTerm densityTerm(const PbfConsts& c, float4 pb, float px, float py,
-                                          float pz) {
-  const float dx = px - pb.x, dy = py - pb.y, dz = pz - pb.z;
-  const float t = fmaxf(__builtin_fmaf(-dx, dx, __builtin_fmaf(-dy, dy, __builtin_fmaf(-dz, dz, c.hh))),
-                        0.0f);
-  return DensityTerm{pb.w * t, t * t};
+// The x and y part of a candidate's distance term, shared by a column.
+__device__ inline float columnPart(float hh, float4 q, float px, float py) {
+  const float dx = px - q.x, dy = py - q.y;
+  return __builtin_fmaf(-dy, dy, __builtin_fmaf(-dx, dx, hh));
 }
-__device__ inline void addTerm(float& rho, DensityTerm d) { rho = __builtin_fmaf(d.wt, d.tt, rho); }
+// One sample's term from the shared part.
+__device__ inline void addTerm(float& rho, float u, float4 q, float pz) {
+  const float dz = pz - q.z;
+  const float t = fmaxf(__builtin_fmaf(-dz, dz, u), 0.0f);
+  rho = __builtin_fmaf(q.w * t, t * t, rho);
+}
+
+// The terms of staged candidates [b, e) for the column's samples KLO..KHI.
+template <int KLO, int KHI, int kNZ>
+__device__ inline void rowTerms(const float4* __restrict__ staged, uint32_t b, uint32_t e, float hh,
+                                float px, float py, const float (&pz)[kNZ], float (&rho)[kNZ]) {
+  for (; b + kUnroll <= e; b += kUnroll) {
+    float4 q[kUnroll];
+#pragma unroll
+    for (uint32_t v = 0; v < kUnroll; ++v) q[v] = staged[b + v];
+#pragma unroll
+    for (uint32_t v = 0; v < kUnroll; ++v) {
+      const float u = columnPart(hh, q[v], px, py);
+#pragma unroll
+      for (int k = KLO; k <= KHI; ++k) addTerm(rho[k], u, q[v], pz[k]);
+    }
+  }
+  for (; b < e; ++b) {
+    const float4 q = staged[b];
+    const float u = columnPart(hh, q, px, py);
+#pragma unroll
+    for (int k = KLO; k <= KHI; ++k) addTerm(rho[k], u, q, pz[k]);
+  }
+}
+
+// Dispatch on the (uniform) range of samples a row belongs to.
+template <int kNZ, int KLO = 0, int KHI = 0>
+__device__ inline void rowTermsFor(int lo, int hi, const float4* __restrict__ staged, uint32_t b,
+                                   uint32_t e, float hh, float px, float py,
+                                   const float (&pz)[kNZ], float (&rho)[kNZ]) {
+  if constexpr (KLO < kNZ) {
+    if constexpr (KHI < kNZ) {
+      if (lo == KLO && hi == KHI) {
+        rowTerms<KLO, KHI, kNZ>(staged, b, e, hh, px, py, pz, rho);
+        return;
+      }
+      rowTermsFor<kNZ, KLO, KHI + 1>(lo, hi, staged, b, e, hh, px, py, pz, rho);
+    } else {
+      rowTermsFor<kNZ, KLO + 1, KLO + 1>(lo, hi, staged, b, e, hh, px, py, pz, rho);
+    }
+  }
+}
 
 __device__ inline void cellCoords(const PbfConsts& c, float px, float py, float pz, uint32_t& x,
                                   uint32_t& y, uint32_t& z) {
@@ -93,21 +137,23 @@ __device__ inline void cellCoords(const PbfConsts& c, float px, float py, float 
   z = cellCoord((pz - c.minExt[2]) / c.h);
 }
 
-template <int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
-                                                         const uint32_t* __restrict__ table,
-                                                         const float4* __restrict__ P,
-                                                         float* __restrict__ field,
-                                                         uint32_t tilesY, uint32_t tilesZ,
-                                                         const uint32_t* __restrict__ health) {
-  static_assert(kTX * kTY * kTZ == kThreads, "one sample per thread");
+// Tile: kWX x kWY columns (one wave) x kNZ * kWaves samples along z (the
+// waves stacked in z); kCap staged particles.
+template <int kWX, int kWY, int kNZ, int kWaves, int kCap>
+__global__ __launch_bounds__(64 * kWaves) void k_mc_density(Lattice L, PbfConsts c,
+                                                             const uint32_t* __restrict__ table,
+                                                             const float4* __restrict__ P,
+                                                             float* __restrict__ field,
+                                                             uint32_t tilesY, uint32_t tilesZ,
+                                                             const uint32_t* __restrict__ health) {
+  static_assert(kWX * kWY == 64, "a wave is a kWX x kWY block of columns");
+  constexpr uint32_t kT = 64 * kWaves;
+  constexpr uint32_t kTX = kWX, kTY = kWY, kTZ = kNZ * kWaves;
+  constexpr uint32_t kRounds = (kHaloCells + kT - 1) / kT;  // halo cells per thread
   if (health[0] | health[1]) return;  // a diverged state: no field
-  // staged particles as 16-byte (x, y, z, m) records: one LDS read per
-  // candidate at a constant offset from one address (separate x/y/z/m arrays
-  // 8 KB apart need extra address arithmetic)
-  __shared__ float4 staged[kStageCap];
+  __shared__ float4 staged[kCap];  // (x, y, z, m poly6): one LDS read per candidate
   __shared__ uint32_t cellStart[kHaloCells + 1];
-  __shared__ uint32_t scanTmp[kThreads / 64];
+  __shared__ uint32_t scanTmp[kWaves];
   __shared__ int mode;  // 0 LDS, 1 global
   const uint32_t tz = blockIdx.x % tilesZ;
   const uint32_t ty = (blockIdx.x / tilesZ) % tilesY;
@@ -115,6 +161,16 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
   const uint32_t x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
   const uint32_t x1 = min(x0 + kTX, L.S[0]) - 1, y1 = min(y0 + kTY, L.S[1]) - 1,
                  z1 = min(z0 + kTZ, L.S[2]) - 1;
+  // the lane's column
+  const uint32_t w = threadIdx.x >> 6, li = threadIdx.x & 63u;
+  const uint32_t x = x0 + li / kWY, y = y0 + li % kWY, zb = z0 + w * kNZ;
+  const bool inside = x < L.S[0] && y < L.S[1] && zb < L.S[2];
+  const uint64_t col = (uint64_t(x) * L.S[1] + y) * L.S[2] + zb;
+  auto writeColumn = [&](const float (&v)[kNZ]) {
+#pragma unroll
+    for (int k = 0; k < kNZ; ++k)
+      if (zb + k < L.S[2]) field[col + k] = v[k];
+  };
   // cell range of the tile (cellCoord is monotone in the sample position)
   uint32_t lo[3], hi[3];
   cellCoords(c, __builtin_fmaf(float(x0), L.step[0], L.origin[0]),
@@ -137,12 +193,14 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
     // Quick emptiness test: an aligned 4x4x4 block of cells is one run of 64
     // Morton codes, so table[end] - table[start] counts its particles (codes
     // up to G - 1, the cells that hold any). A tile whose halo lies in blocks
-    // holding none writes zeros without staging anything.
+    // holding none writes zeros without staging anything. (More blocks than
+    // threads: no test.)
     const uint32_t b0[3] = {(lo[0] - 1) >> 2, (lo[1] - 1) >> 2, (lo[2] - 1) >> 2};
     const uint32_t nb[3] = {((hi[0] + 1) >> 2) - b0[0] + 1, ((hi[1] + 1) >> 2) - b0[1] + 1,
                             ((hi[2] + 1) >> 2) - b0[2] + 1};
-    bool any = false;
-    if (threadIdx.x < nb[0] * nb[1] * nb[2]) {
+    const uint32_t nbl = nb[0] * nb[1] * nb[2];
+    bool any = nbl > kT;
+    if (threadIdx.x < nbl && !any) {
       const uint32_t i = threadIdx.x;
       const uint32_t base = mortonEncode(4u * (b0[0] + i % nb[0]), 4u * (b0[1] + (i / nb[0]) % nb[1]),
                                          4u * (b0[2] + i / (nb[0] * nb[1])));
@@ -150,71 +208,65 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
       any = table[min(base + 64u, last)] != table[min(base, last)];
     }
     if (!__syncthreads_or(any)) {
-      const uint32_t lx = threadIdx.x / (kTY * kTZ), ly = (threadIdx.x / kTZ) % kTY,
-                     lz = threadIdx.x % kTZ;
-      const uint32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
-      if (x < L.S[0] && y < L.S[1] && z < L.S[2])
-        field[(uint64_t(x) * L.S[1] + y) * L.S[2] + z] = 0.0f;
+      if (inside) {
+        float z[kNZ];
+#pragma unroll
+        for (int k = 0; k < kNZ; ++k) z[k] = 0.0f;
+        writeColumn(z);
+      }
       return;
     }
   }
-  // count the halo cells' particles (per thread: cells tid, tid+256)
-  uint32_t myCount[2] = {0, 0};
-  for (int r = 0; r < 2; ++r) {
-    const uint32_t h = threadIdx.x + r * kThreads;
+  // count the halo cells' particles (per thread: cells tid + r * kT)
+  uint32_t myCount[kRounds];
+#pragma unroll
+  for (uint32_t r = 0; r < kRounds; ++r) {
+    myCount[r] = 0;
+    const uint32_t h = threadIdx.x + r * kT;
     if (h < nh) {
       const uint32_t ix = h % hx, iy = (h / hx) % hy, iz = h / (hx * hy);
       const uint32_t code = mortonEncode(lo[0] - 1 + ix, lo[1] - 1 + iy, lo[2] - 1 + iz);
       if (code + 1u < c.G) myCount[r] = table[code + 1u] - table[code];
     }
   }
-  // exclusive scan over cells in order h = 0..nh-1 (cells tid and tid+256)
-  uint32_t total0, total1;
+  // exclusive scan over cells in order h = 0..nh-1, one round of kT cells at
+  // a time
+  uint32_t totalParticles = 0;
   {
-    const int lane = __lane_id(), w = threadIdx.x >> 6;
-    uint32_t v = myCount[0], x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; ++r) {
+      const uint32_t v = myCount[r];
+      uint32_t xs = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t yv = __shfl_up(xs, o, 64);
+        if (lane >= uint32_t(o)) xs += yv;
+      }
+      if (lane == 63u) scanTmp[w] = xs;
+      __syncthreads();
+      uint32_t wb = 0, tot = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < kWaves; ++i) {
+        if (i < w) wb += scanTmp[i];
+        tot += scanTmp[i];
+      }
+      const uint32_t h = threadIdx.x + r * kT;
+      if (h < nh) cellStart[h] = totalParticles + wb + xs - v;
+      totalParticles += tot;
+      __syncthreads();
     }
-    if (lane == 63) scanTmp[w] = x;
-    __syncthreads();
-    uint32_t wb = 0;
-    total0 = 0;
-    for (int i = 0; i < kThreads / 64; ++i) {
-      if (i < w) wb += scanTmp[i];
-      total0 += scanTmp[i];
-    }
-    const uint32_t ex0 = wb + x - v;
-    __syncthreads();
-    v = myCount[1];
-    x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) scanTmp[w] = x;
-    __syncthreads();
-    wb = 0;
-    total1 = 0;
-    for (int i = 0; i < kThreads / 64; ++i) {
-      if (i < w) wb += scanTmp[i];
-      total1 += scanTmp[i];
-    }
-    const uint32_t ex1 = total0 + wb + x - v;
-    if (threadIdx.x < nh) cellStart[threadIdx.x] = ex0;
-    if (threadIdx.x + kThreads < nh) cellStart[threadIdx.x + kThreads] = ex1;
     if (threadIdx.x == 0) {
-      cellStart[nh] = total0 + total1;
-      mode = (!fits || total0 + total1 > uint32_t(kStageCap)) ? 1 : 0;
+      cellStart[nh] = totalParticles;
+      mode = (!fits || totalParticles > uint32_t(kCap)) ? 1 : 0;
     }
   }
   __syncthreads();
-  const uint32_t totalParticles = total0 + total1;
   const bool useLds = mode == 0;
   if (useLds) {
-    for (int r = 0; r < 2; ++r) {
-      const uint32_t h = threadIdx.x + r * kThreads;
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; ++r) {
+      const uint32_t h = threadIdx.x + r * kT;
       if (h < nh && myCount[r] > 0) {
         const uint32_t ix = h % hx, iy = (h / hx) % hy, iz = h / (hx * hy);
         const uint32_t code = mortonEncode(lo[0] - 1 + ix, lo[1] - 1 + iy, lo[2] - 1 + iz);
@@ -227,57 +279,66 @@ __global__ __launch_bounds__(kThreads) void k_mc_density(Lattice L, PbfConsts c,
           for (uint32_t u = 0; u < 4; ++u) q[u] = P[s0 + min(j + u, cnt - 1u)];
 #pragma unroll
           for (uint32_t u = 0; u < 4; ++u)
-            if (j + u < cnt) {
-              staged[dst + j + u] = make_float4(q[u].x, q[u].y, q[u].z, q[u].w * c.poly6);
-            }
+            if (j + u < cnt) staged[dst + j + u] = make_float4(q[u].x, q[u].y, q[u].z, q[u].w * c.poly6);
         }
       }
     }
   }
   __syncthreads();
-  const uint32_t lx = threadIdx.x / (kTY * kTZ), ly = (threadIdx.x / kTZ) % kTY,
-                 lz = threadIdx.x % kTZ;
-  const uint32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
-  if (x >= L.S[0] || y >= L.S[1] || z >= L.S[2]) return;
-  const uint64_t idx = (uint64_t(x) * L.S[1] + y) * L.S[2] + z;
+  if (!inside) return;
+  // samples past the lattice's end repeat its last one (computed, not written),
+  // so every sample's cells lie in the tile's range
+  float rho[kNZ], pz[kNZ];
+#pragma unroll
+  for (int k = 0; k < kNZ; ++k) {
+    rho[k] = 0.0f;
+    pz[k] = __builtin_fmaf(float(min(zb + uint32_t(k), L.S[2] - 1u)), L.step[2], L.origin[2]);
+  }
   if (useLds && totalParticles == 0) {
-    field[idx] = 0.0f;
+    writeColumn(rho);
     return;
   }
   const float px = __builtin_fmaf(float(x), L.step[0], L.origin[0]);
   const float py = __builtin_fmaf(float(y), L.step[1], L.origin[1]);
-  const float pz = __builtin_fmaf(float(z), L.step[2], L.origin[2]);
-  float rho = 0.0f;
   if (useLds) {
-    uint32_t cx, cy, cz;
-    cellCoords(c, px, py, pz, cx, cy, cz);
-    const uint32_t bx = cx - lo[0], by = cy - lo[1], bz = cz - lo[2];  // halo index of dx=-1
-    for (uint32_t dz = 0; dz < 3; ++dz)
+    uint32_t cx, cy, cz0;
+    cellCoords(c, px, py, pz[0], cx, cy, cz0);
+    // the z cells of the column's samples: the same on every lane of the
+    // wave (they depend on z only); halo z index of each sample's dz = -1 row
+    int zl[kNZ];
+#pragma unroll
+    for (int k = 0; k < kNZ; ++k)
+      zl[k] = __builtin_amdgcn_readfirstlane(int(cellCoord((pz[k] - c.minExt[2]) / c.h) - lo[2]));
+    const uint32_t bx = cx - lo[0], by = cy - lo[1];  // halo index of dx = -1, dy = -1
+    for (int iz = zl[0]; iz <= zl[kNZ - 1] + 2; ++iz) {
+      // samples whose rows include halo z row iz: zl[k] <= iz <= zl[k] + 2
+      int klo = kNZ, khi = -1;
+#pragma unroll
+      for (int k = 0; k < kNZ; ++k)
+        if (zl[k] <= iz && iz <= zl[k] + 2) {
+          klo = min(klo, k);
+          khi = max(khi, k);
+        }
       for (uint32_t dy = 0; dy < 3; ++dy) {
         // the three x cells of a row are consecutive halo cells, so their
         // particles are one contiguous run of the staging, in visit order
-        const uint32_t row = ((bz + dz) * hy + (by + dy)) * hx + bx;
-        const uint32_t e = cellStart[row + 3];
-        uint32_t b = cellStart[row];
-        // kUnroll staged particles in flight, terms added in order
-        for (; b + kUnroll <= e; b += kUnroll) {
-          DensityTerm t[kUnroll];
-#pragma unroll
-          for (uint32_t u = 0; u < kUnroll; ++u) t[u] = densityTerm(c, staged[b + u], px, py, pz);
-#pragma unroll
-          for (uint32_t u = 0; u < kUnroll; ++u) addTerm(rho, t[u]);
-        }
-        for (; b < e; ++b) addTerm(rho, densityTerm(c, staged[b], px, py, pz));
+        const uint32_t row = (uint32_t(iz) * hy + (by + dy)) * hx + bx;
+        rowTermsFor<kNZ>(klo, khi, staged, cellStart[row], cellStart[row + 3], c.hh, px, py, pz,
+                         rho);
       }
+    }
   } else {
-    forEachNeighbourCell(cellKey(c, px, py, pz), table, c.G, [&](uint32_t s, uint32_t e) {
-      for (uint32_t b = s; b < e; ++b) {
-        const float4 pb = P[b];
-        addTerm(rho, densityTerm(c, make_float4(pb.x, pb.y, pb.z, pb.w * c.poly6), px, py, pz));
-      }
-    });
+#pragma unroll
+    for (int k = 0; k < kNZ; ++k)
+      forEachNeighbourCell(cellKey(c, px, py, pz[k]), table, c.G, [&](uint32_t s, uint32_t e) {
+        for (uint32_t b = s; b < e; ++b) {
+          const float4 pb = P[b];
+          const float4 q = make_float4(pb.x, pb.y, pb.z, pb.w * c.poly6);
+          addTerm(rho[k], columnPart(c.hh, q, px, py), q, pz[k]);
+        }
+      });
   }
-  field[idx] = rho;
+  writeColumn(rho);
 }
 
 // mc_lattice (oclsph_kernel.h:421-530): metaball field sum of size/|l|^infl,
@@ -755,19 +816,29 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
     c->fieldOnMain = true;
     c->mcAttr = 1;
   } else {
-    static const int kShapes[3][3] = {{8, 8, 4}, {4, 8, 8}, {8, 4, 8}};
-    const int* sh = kShapes[c->mcTile < 3 ? c->mcTile : 0];
-    const uint32_t kTX = uint32_t(sh[0]), kTY = uint32_t(sh[1]), kTZ = uint32_t(sh[2]);
-    const uint32_t tX = (L.S[0] + kTX - 1) / kTX, tY = (L.S[1] + kTY - 1) / kTY,
-                   tZ = (L.S[2] + kTZ - 1) / kTZ;
+    // tile shapes (lf_exec_params.mc_tile): columns kWX x kWY per wave,
+    // kNZ samples per column, waves stacked in z
+    struct Shape {
+      uint32_t tx, ty, tz, threads;
+    };
+    static const Shape kShapes[kMcTiles] = {{8, 8, 16, 256}, {4, 16, 16, 256}, {8, 8, 8, 256},
+                                            {8, 8, 4, 256}};
+    const Shape sh = kShapes[c->mcTile < kMcTiles ? c->mcTile : 0];
+    const uint32_t tX = (L.S[0] + sh.tx - 1) / sh.tx, tY = (L.S[1] + sh.ty - 1) / sh.ty,
+                   tZ = (L.S[2] + sh.tz - 1) / sh.tz;
     const uint64_t tiles = uint64_t(tX) * tY * tZ;
     if (tiles >= (uint64_t(1) << 31)) return fail(c, LF_ERR_ARG, "lf_mc_sample: lattice too large");
     // on the marching-cubes stream, after the step: it reads only P and the
     // table, which the next step leaves alone (it writes the other pair)
     OnMcStream onMc(c, McInput::Positions);
-    auto kern = c->mcTile == 1 ? k_mc_density<4, 8, 8>
-                : c->mcTile == 2 ? k_mc_density<8, 4, 8> : k_mc_density<8, 8, 4>;
-    rc = launch(c, "mc_density_field", kern, dim3(unsigned(tiles)), dim3(kThreads), 0, L, k, table,
+    // 2,048 staged particles (32 KB): the halos of the 1M dam break's 256^3
+    // tiles hold at most 1,932 (8 x 8 x 16; median 1,663); a tile above it
+    // reads global memory
+    auto kern = c->mcTile == 1   ? k_mc_density<4, 16, 4, 4, 2048>
+                : c->mcTile == 2 ? k_mc_density<8, 8, 2, 4, 2048>
+                : c->mcTile == 3 ? k_mc_density<8, 8, 1, 4, 2048>
+                                 : k_mc_density<8, 8, 4, 4, 2048>;
+    rc = launch(c, "mc_density_field", kern, dim3(unsigned(tiles)), dim3(sh.threads), 0, L, k, table,
                 (const float4*)c->P.p, c->field.p, tY, tZ, (const uint32_t*)c->health.p);
     c->lastDensity = m;
     c->haveDensity = true;

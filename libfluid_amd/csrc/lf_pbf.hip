@@ -32,16 +32,19 @@ namespace lf {
 namespace {
 
 #ifndef LF_BLOCK_LAMBDA
-#define LF_BLOCK_LAMBDA 16
+#define LF_BLOCK_LAMBDA 8
 #endif
 #ifndef LF_BLOCK_DELTA
-#define LF_BLOCK_DELTA 16
+#define LF_BLOCK_DELTA 8
 #endif
 #ifndef LF_BLOCK_XSPH
-#define LF_BLOCK_XSPH 16
+#define LF_BLOCK_XSPH 8
 #endif
-// list entries in flight per lane (forEachAccepted); 16 measured against 8 at
-// 1M: XSPH 211 -> 193 us, FAST_MATH lambda 104 -> 93 us, the combined batch -9 %
+// list entries in flight per lane (forEachAccepted). 8 for all three at 1M
+// with the density field beside the step: 1.789 -> 1.720 ms/step over steps
+// 5-25, 1.583 -> 1.488 over steps 5-105 (XSPH, 127 -> fewer VGPRs and more
+// waves per SIMD, 288 -> 215 us); 16 had been faster when the passes ran
+// alone (XSPH 211 -> 193 us).
 constexpr int kBlockLambda = LF_BLOCK_LAMBDA;
 constexpr int kBlockDelta = LF_BLOCK_DELTA;
 constexpr int kBlockXsph = LF_BLOCK_XSPH;

@@ -621,8 +621,8 @@ template <typename F> class Pbf {
 
   /* SPH density field rho(x) = sum over fluid b of m_b poly6 (h^2 - d^2)^3 for
    * |x - p*_b| < h, evaluated as r = fma((m_b poly6) t, t t, r) with
-   * t = h^2 - dz^2 - dy^2 - dx^2 taken as one fused multiply-add chain from
-   * h^2, on the lattice
+   * t = h^2 - dx^2 - dy^2 - dz^2 taken as one fused multiply-add chain from
+   * h^2 (x first), on the lattice
    * x_ijk = origin + (i,j,k) * step (solver units). Candidates: the 27 cells
    * around the sample's own cell, as the PBF passes; terms added in order. */
   void mcDensityField(const uint32_t S[3], const F origin[3], const F step[3],
@@ -641,7 +641,7 @@ template <typename F> class Pbf {
             if (ts[b] == LF_OBSTACLE) return;
             const F* pb = &ps[4 * b];
             const F dx = p[0] - pb[0], dy = p[1] - pb[1], dz = p[2] - pb[2];
-            const F t = std::fma(-dx, dx, std::fma(-dy, dy, std::fma(-dz, dz, c.hh)));
+            const F t = std::fma(-dz, dz, std::fma(-dy, dy, std::fma(-dx, dx, c.hh)));
             if (!(t > F(0))) return;
             r = std::fma((pb[3] * c.poly6) * t, t * t, r);
           });

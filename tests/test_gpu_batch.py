@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import libfluid_amd as lf
+import oracle
 from libfluid_amd import scenes
 from libfluid_amd._abi import LF_ARR_CELLKEY, LF_ARR_FIELD, LF_ARR_KEYS, LF_ARR_TABLE
 
@@ -252,23 +253,33 @@ def test_batch_rejects_mixed_iterations(ctx):
 
 
 def test_density_tile_shapes_same_field():
-    """The density field's sample tile (a runtime choice) changes nothing: the
-    same terms in the same order per sample, whether staged in LDS or read from
-    global memory; the autotuner times the shapes on the last density lattice."""
+    """The density field's sample tile (a runtime choice: columns of 1, 2 or 4
+    samples per thread) changes nothing: the same terms in the same order per
+    sample, whether staged in LDS or read from global memory, on a cubic
+    lattice and on a ragged one whose tiles are cut at every edge (equal to
+    the oracle's FP32 field there too); the autotuner times the shapes on the
+    last density lattice."""
     sc = scenes.dam_break_8k()
     p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
-    m = sc.mc()
-    fields = []
-    for tile in (0, 1, 2):
+    lattices = (sc.mc(), lf.mc_density_params(sc.isolevel, (61, 45, 37)))
+    fields = {}
+    for tile in range(4):
         with lf.Context(0) as c:
             c.exec_config(256, False, 0.2, mc_tile=tile)
             c.upload(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
             c.step(p, 2)
-            c.mc_sample(p, m)
-            fields.append(c.read(LF_ARR_FIELD))
+            for j, m in enumerate(lattices):
+                c.mc_sample(p, m)
+                fields[tile, j] = c.read(LF_ARR_FIELD)
             if tile == 2:
                 r = c.autotune(1)
-                assert r["trials"] == 39 and r["mc_tile"] in (0, 1, 2) and r["mc_ms"] > 0
-    assert (fields[0] > 0).sum() > 1000
-    _same(fields[1], fields[0], "tile 4x8x8")
-    _same(fields[2], fields[0], "tile 8x4x8")
+                assert r["trials"] == 40 and r["mc_tile"] in range(4) and r["mc_ms"] > 0
+    o = oracle.Pbf(p)
+    o.load(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
+    o.step(2)
+    ragged = o.mc_density(lattices[1].samples)
+    for j in range(2):
+        assert (fields[0, j] > 0).sum() > 1000
+        for tile in (1, 2, 3):
+            _same(fields[tile, j], fields[0, j], "tile %d, lattice %d" % (tile, j))
+    _same(fields[0, 1].astype(np.float64).ravel(), ragged.ravel(), "ragged lattice against the oracle")
