@@ -398,7 +398,7 @@ def main():
                     help="skip the 32-scene ~8M combined case")
     ap.add_argument("--skin", type=float, default=None,
                     help="neighbour-list skin (fraction of h) of the 1M runs (default: the "
-                         "library's, 0.2)")
+                         "library's, 0.15)")
     ap.add_argument("--persistent", action="store_true",
                     help="work-queue launches for the pair kernels of the 1M runs")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -438,7 +438,7 @@ def main():
 
     ctx = lf.Context(0)
     if args.skin is not None or args.persistent:
-        ctx.exec_config(256, args.persistent, 0.2 if args.skin is None else args.skin)
+        ctx.exec_config(256, args.persistent, 0.15 if args.skin is None else args.skin)
     ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
 
     def one_step():

@@ -309,7 +309,7 @@ class Context:
                     density_error_max=d.density_error_max, passes=d.passes,
                     pass_error=list(d.pass_error)[:max(0, d.passes)])
 
-    def exec_config(self, threads=256, persistent=False, skin=0.2, particles_per_thread=1,
+    def exec_config(self, threads=256, persistent=False, skin=0.15, particles_per_thread=1,
                     mc_tile=0):
         e = _abi.ExecParams(threads, 1 if persistent else 0, skin, particles_per_thread, mc_tile)
         self._check(self._lib.lf_exec_configure(self._h, ctypes.byref(e)))

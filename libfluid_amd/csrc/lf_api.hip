@@ -1230,7 +1230,7 @@ int lf_autotune(lf_ctx* c, uint32_t steps, lf_tune_result* out) {
   const uint32_t nQueries = c->nQueries;
   c->nQueries = 0;
   const uint32_t threadsList[3] = {128, 256, 512};
-  const float skins[3] = {0.1f, 0.2f, 0.3f};
+  const float skins[4] = {0.1f, 0.15f, 0.2f, 0.3f};
   // (particles per thread, persistent): the queue takes chunks one at a time
   const uint32_t pptPers[4][2] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}};
   lf_tune_result best{};

@@ -607,7 +607,7 @@ struct Ctx {
 
   bool gridValid = false;
   bool stepDone = false;
-  float skinFrac = 0.2f;   // neighbour-list skin / h
+  float skinFrac = 0.15f;  // neighbour-list skin / h
   bool listFresh = false;  // P still holds the predicted p* the lists were built from
   PbfConsts consts{};      // scene 0 (single-scene calls), or the scene of lf_mc_* calls
   lf_pbf_params params{};

@@ -209,7 +209,7 @@ def test_autotune_keeps_state():
         c.step(p, 1)
         before = c.download()
         r = c.autotune(2)
-        assert r["trials"] == 36 and r["threads"] in (128, 256, 512) and r["ms_per_step"] > 0
+        assert r["trials"] == 48 and r["threads"] in (128, 256, 512) and r["ms_per_step"] > 0
         assert r["particles_per_thread"] in (1, 2, 4) and r["mc_tile"] == 0 and r["mc_ms"] == 0
         after = c.download()
         for k in ("position", "velocity", "colour", "order"):
@@ -273,7 +273,7 @@ def test_density_tile_shapes_same_field():
                 fields[tile, j] = c.read(LF_ARR_FIELD)
             if tile == 2:
                 r = c.autotune(1)
-                assert r["trials"] == 40 and r["mc_tile"] in range(4) and r["mc_ms"] > 0
+                assert r["trials"] == 52 and r["mc_tile"] in range(4) and r["mc_ms"] > 0
     o = oracle.Pbf(p)
     o.load(sc.position, sc.velocity, sc.mass, sc.type, sc.colour)
     o.step(2)
