@@ -832,6 +832,7 @@ int lf_mc_field_upload(lf_ctx* c, const uint32_t samples[3], const float origin[
   if (rc) return rc;
   LF_HIP(c, hipMemcpyAsync(c->field.p, field, total * sizeof(float), hipMemcpyHostToDevice, c->stream));
   c->fieldOnMain = true;
+  c->mcTileFlags = false;
   LF_HIP(c, hipStreamSynchronize(c->stream));
   for (int k = 0; k < 3; ++k) {
     c->S[k] = samples[k];
@@ -976,7 +977,10 @@ int lf_write(lf_ctx* c, int array, const void* src, size_t bytes) {
   // event no longer covers P; a written field is the main stream's
   if (lf::mcJoin(c)) return lf::fail(c, LF_ERR_HIP, "waiting for the marching-cubes stream");
   c->posEventValid = false;
-  if (array == LF_ARR_FIELD) c->fieldOnMain = true;
+  if (array == LF_ARR_FIELD) {
+    c->fieldOnMain = true;
+    c->mcTileFlags = false;
+  }
   LF_HIP(c, hipSetDevice(c->device));
   void* dst = nullptr;
   size_t avail = 0;

@@ -667,6 +667,12 @@ struct Ctx {
   int mcAttr = 0;
   bool keepCases = false;
   uint32_t* mcScratch = nullptr;  // lf_mc_mesh_static's block counts on this device
+  // per tile of the last density field: 1 = every sample of the tile was
+  // written as zero (its halo holds no particle); valid while the field is
+  // that density field (mcTileFlags), with the tile geometry it was sampled on
+  DevBuf<uint8_t> mcTileEmpty;
+  bool mcTileFlags = false;
+  uint32_t mcTileSize[3] = {0, 0, 0}, mcTiles[3] = {0, 0, 0};
   float mcIso = 0.0f;  // isolevel of the last classify pass (its cases feed the emit)
 
   // profiling

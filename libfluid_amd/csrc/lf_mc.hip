@@ -145,7 +145,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_mc_density(Lattice L, PbfConsts
                                                              const float4* __restrict__ P,
                                                              float* __restrict__ field,
                                                              uint32_t tilesY, uint32_t tilesZ,
-                                                             const uint32_t* __restrict__ health) {
+                                                             const uint32_t* __restrict__ health,
+                                                             uint8_t* __restrict__ tileEmpty) {
   static_assert(kWX * kWY == 64, "a wave is a kWX x kWY block of columns");
   constexpr uint32_t kT = 64 * kWaves;
   constexpr uint32_t kTX = kWX, kTY = kWY, kTZ = kNZ * kWaves;
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mc_density(Lattice L, PbfConsts
       any = table[min(base + 64u, last)] != table[min(base, last)];
     }
     if (!__syncthreads_or(any)) {
+      if (threadIdx.x == 0) tileEmpty[blockIdx.x] = 1u;
       if (inside) {
         float z[kNZ];
 #pragma unroll
@@ -285,6 +287,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mc_density(Lattice L, PbfConsts
     }
   }
   __syncthreads();
+  if (threadIdx.x == 0) tileEmpty[blockIdx.x] = 0u;
   if (!inside) return;
   // samples past the lattice's end repeat its last one (computed, not written),
   // so every sample's cells lie in the tile's range
@@ -475,10 +478,48 @@ __device__ inline uint32_t cubeCase(const CubeGrid& g, const float* __restrict__
 // mc_size (oclsph_kernel.h:545-596): triangles per block of kCubesPerBlock
 // cubes. Cube m = block*1024 + j*256 + thread (j = 0..3). The case indices
 // are kept (one byte per cube) when cases is not null.
+// With tile flags of the density field that wrote the field: a block whose
+// samples all lie in tiles written as zero has a uniform field, so every cube
+// has 0 triangles (case 0 or 255) and the block writes a zero count without
+// reading the field (its case bytes are not written: the emission never
+// reads them for a block without triangles). (Testing the flags per cube
+// instead of reading its 8 samples was slower: 88 -> 108 us at 256^3.)
+struct TileFlags {
+  const uint8_t* empty;  // null: no flags
+  uint32_t size[3];      // samples per tile
+  uint32_t tiles[3];     // tiles per axis
+};
 __global__ __launch_bounds__(kThreads) void k_mc_count(CubeGrid g, const float* __restrict__ field,
                                                        float iso, uint32_t* __restrict__ blockCount,
-                                                       uint8_t* __restrict__ cases) {
+                                                       uint8_t* __restrict__ cases, TileFlags tf) {
   __shared__ uint32_t sm[kThreads / 64];
+  if (tf.empty) {
+    // the block's cubes [c0, c1]: samples x0..x1+1 and, within one x, y0..y1+1,
+    // within one (x, y) z0..z1+1; else the whole y or z range
+    const uint32_t c0 = blockIdx.x * kCubesPerBlock,
+                   c1 = min(c0 + kCubesPerBlock, g.cubes) - 1u;
+    const uint32_t x0 = c0 / (g.M[1] * g.M[2]), x1 = c1 / (g.M[1] * g.M[2]);
+    const uint32_t y0 = (c0 / g.M[2]) % g.M[1], y1 = (c1 / g.M[2]) % g.M[1];
+    const uint32_t sy0 = x0 == x1 ? y0 : 0u, sy1 = x0 == x1 ? y1 + 1u : g.S[1] - 1u;
+    const bool oneColumn = x0 == x1 && y0 == y1;
+    const uint32_t sz0 = oneColumn ? c0 % g.M[2] : 0u,
+                   sz1 = oneColumn ? c1 % g.M[2] + 1u : g.S[2] - 1u;
+    const uint32_t t0[3] = {x0 / tf.size[0], sy0 / tf.size[1], sz0 / tf.size[2]};
+    const uint32_t n[3] = {(x1 + 1u) / tf.size[0] - t0[0] + 1u, sy1 / tf.size[1] - t0[1] + 1u,
+                           sz1 / tf.size[2] - t0[2] + 1u};
+    const uint32_t nt = n[0] * n[1] * n[2];
+    bool empty = nt <= uint32_t(kThreads);  // more tiles than threads: no test
+    if (empty && threadIdx.x < nt) {
+      const uint32_t i = threadIdx.x;
+      const uint32_t tx = t0[0] + i / (n[1] * n[2]), ty = t0[1] + (i / n[2]) % n[1],
+                     tz = t0[2] + i % n[2];
+      empty = tf.empty[(uint64_t(tx) * tf.tiles[1] + ty) * tf.tiles[2] + tz] != 0;
+    }
+    if (__syncthreads_and(empty)) {
+      if (threadIdx.x == 0) blockCount[blockIdx.x] = 0u;
+      return;
+    }
+  }
   uint32_t sum = 0;
   const uint32_t m0 = blockIdx.x * kCubesPerBlock + threadIdx.x;
   CubeCursor q(g, min(m0, g.cubes - 1u));
@@ -814,6 +855,7 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
                 (const float4*)c->V[c->cur].p, c->field.p, c->mcNormal.p, c->mcColour.p,
                 (const uint32_t*)c->health.p);
     c->fieldOnMain = true;
+    c->mcTileFlags = false;
     c->mcAttr = 1;
   } else {
     // tile shapes (lf_exec_params.mc_tile): columns kWX x kWY per wave,
@@ -838,8 +880,18 @@ int mcSample(Ctx* c, const lf_mc_params& m) {
                 : c->mcTile == 2 ? k_mc_density<8, 8, 2, 4, 2048>
                 : c->mcTile == 3 ? k_mc_density<8, 8, 1, 4, 2048>
                                  : k_mc_density<8, 8, 4, 4, 2048>;
+    if ((rc = ensure(c, c->mcTileEmpty, size_t(tiles), "mc tile flags"))) return rc;
     rc = launch(c, "mc_density_field", kern, dim3(unsigned(tiles)), dim3(sh.threads), 0, L, k, table,
-                (const float4*)c->P.p, c->field.p, tY, tZ, (const uint32_t*)c->health.p);
+                (const float4*)c->P.p, c->field.p, tY, tZ, (const uint32_t*)c->health.p,
+                c->mcTileEmpty.p);
+    // a diverged state returns before writing field or flags: no flags then
+    c->mcTileFlags = rc == LF_OK;
+    c->mcTileSize[0] = sh.tx;
+    c->mcTileSize[1] = sh.ty;
+    c->mcTileSize[2] = sh.tz;
+    c->mcTiles[0] = tX;
+    c->mcTiles[1] = tY;
+    c->mcTiles[2] = tZ;
     c->lastDensity = m;
     c->haveDensity = true;
     if (rc == LF_OK && onMc.on) {
@@ -883,8 +935,16 @@ int mcClassifyCount(Ctx* c, const lf_mc_params& m) {
   if ((rc = ensure(c, c->mcTotal, 4, "mc total"))) return rc;
   if ((rc = ensure(c, c->mcCases, std::max<size_t>(g.cubes, 1), "mc cases"))) return rc;
   if (nbl > 0) {
+    TileFlags tf{};
+    if (c->mcTileFlags && !c->keepCases) {
+      tf.empty = c->mcTileEmpty.p;
+      for (int i = 0; i < 3; ++i) {
+        tf.size[i] = c->mcTileSize[i];
+        tf.tiles[i] = c->mcTiles[i];
+      }
+    }
     rc = launch(c, "mc_classify_count", k_mc_count, dim3(unsigned(nbl)), dim3(kThreads), 0, g,
-                (const float*)c->field.p, m.isolevel, c->mcBlock.p, c->mcCases.p);
+                (const float*)c->field.p, m.isolevel, c->mcBlock.p, c->mcCases.p, tf);
     if (rc) return rc;
   }
   c->mcIso = m.isolevel;
@@ -991,7 +1051,7 @@ int mcMeshStatic(Ctx* c, const uint32_t S[3], const float origin[3], const float
   A.attributes = 0;
   // plain launches on the context's stream: no timing events, no buffers
   hipLaunchKernelGGL(k_mc_count, dim3(unsigned(nbl)), dim3(kThreads), 0, c->stream, g, field, iso,
-                     c->mcScratch, (uint8_t*)nullptr);
+                     c->mcScratch, (uint8_t*)nullptr, TileFlags{});
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hipFail(c, e, "mc_classify_count");
   hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, c->stream, c->mcScratch, uint32_t(nbl), ntri);

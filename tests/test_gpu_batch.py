@@ -257,12 +257,14 @@ def test_density_tile_shapes_same_field():
     samples per thread) changes nothing: the same terms in the same order per
     sample, whether staged in LDS or read from global memory, on a cubic
     lattice and on a ragged one whose tiles are cut at every edge (equal to
-    the oracle's FP32 field there too); the autotuner times the shapes on the
-    last density lattice."""
+    the oracle's FP32 field there too). The classification then skips the
+    blocks whose samples all lie in tiles written as zero: the triangles equal
+    the oracle's marching cubes of the field for every tile shape. The
+    autotuner times the shapes on the last density lattice."""
     sc = scenes.dam_break_8k()
     p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
     lattices = (sc.mc(), lf.mc_density_params(sc.isolevel, (61, 45, 37)))
-    fields = {}
+    fields, tris = {}, {}
     for tile in range(4):
         with lf.Context(0) as c:
             c.exec_config(256, False, 0.2, mc_tile=tile)
@@ -270,7 +272,10 @@ def test_density_tile_shapes_same_field():
             c.step(p, 2)
             for j, m in enumerate(lattices):
                 c.mc_sample(p, m)
+                c.mc_classify_count(m)  # skips blocks in tiles the field wrote as zero
+                c.mc_emit(p, m)
                 fields[tile, j] = c.read(LF_ARR_FIELD)
+                tris[tile, j] = c.mc_triangles()
             if tile == 2:
                 r = c.autotune(1)
                 assert r["trials"] == 52 and r["mc_tile"] in range(4) and r["mc_ms"] > 0
@@ -283,3 +288,10 @@ def test_density_tile_shapes_same_field():
         for tile in (1, 2, 3):
             _same(fields[tile, j], fields[0, j], "tile %d, lattice %d" % (tile, j))
     _same(fields[0, 1].astype(np.float64).ravel(), ragged.ravel(), "ragged lattice against the oracle")
+    for j, m in enumerate(lattices):
+        origin, step = o.density_lattice(m.samples)
+        want = oracle.mc_emit(fields[0, j].astype(np.float64).reshape(tuple(m.samples)), m.isolevel,
+                              origin, step, 1.0)
+        assert len(want) > 100
+        for tile in range(4):
+            _same(tris[tile, j]["v"], want["v"], "triangles, tile %d, lattice %d" % (tile, j))
