@@ -368,6 +368,160 @@ __device__ inline void walkCells(const PbfConsts& c, uint32_t key,
   });
 }
 
+#ifndef LF_PAIR_TILES
+#define LF_PAIR_TILES 0
+#endif
+// ---- list-free pair passes over LDS cell tiles (a measured alternative) ----
+// Built with LF_PAIR_TILES=1: the lambda and delta passes (plain FP32) stage
+// the particles of their chunk's halo cells (the chunk's cell box +-1 per
+// axis) in LDS, x fastest, with a local cell table, and each lane walks its
+// 27 cells there: the same candidates in the same order as walkCells, so the
+// sums are bit-identical. A chunk whose halo does not fit, or touches the
+// table's edge, walks its cells in global memory.
+constexpr uint32_t kTileHaloCells = 1024;
+constexpr uint32_t kTileCap = 2048;  // staged records (32 KB)
+
+struct HaloTile {
+  uint32_t lo[3];  // cell coordinates of the halo's first cell
+  uint32_t hx, hy;
+  bool ok;
+};
+
+__device__ inline uint32_t compact10(uint32_t v) {  // inverse of spread10
+  v &= 0x09249249u;
+  v = (v | (v >> 2)) & 0x030C30C3u;
+  v = (v | (v >> 4)) & 0x0300F00Fu;
+  v = (v | (v >> 8)) & 0x030000FFu;
+  v = (v | (v >> 16)) & 0x000003FFu;
+  return v;
+}
+
+// Called by every thread of the workgroup for chunk ch (scene-local keys and
+// table); src: the records the pass gathers (16 bytes each).
+template <int BS>
+__device__ inline HaloTile stageHalo(const PbfConsts& c, const Chunk& ch,
+                                     const uint32_t* __restrict__ keys,
+                                     const uint32_t* __restrict__ table,
+                                     const float4* __restrict__ src, float4* rec, uint32_t* start,
+                                     uint32_t* red) {
+  constexpr uint32_t kW = BS / 64;
+  const uint32_t a = ch.begin + threadIdx.x;
+  const bool valid = a < ch.end;
+  const uint32_t key = valid ? keys[a] : 0u;
+  const uint32_t x = compact10(key), y = compact10(key >> 1), z = compact10(key >> 2);
+  uint32_t mn[3] = {valid ? x : 1023u, valid ? y : 1023u, valid ? z : 1023u};
+  uint32_t mx[3] = {valid ? x : 0u, valid ? y : 0u, valid ? z : 0u};
+  uint32_t bad = (valid && !(key < c.G - 1u && c.G > 0u)) ? 1u : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = min(mn[k], uint32_t(__shfl_xor(int(mn[k]), o, 64)));
+      mx[k] = max(mx[k], uint32_t(__shfl_xor(int(mx[k]), o, 64)));
+    }
+    bad |= uint32_t(__shfl_xor(int(bad), o, 64));
+  }
+  const uint32_t w = threadIdx.x >> 6;
+  if (__lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      red[w * 8 + k] = mn[k];
+      red[w * 8 + 3 + k] = mx[k];
+    }
+    red[w * 8 + 6] = bad;
+  }
+  __syncthreads();
+  HaloTile t{};
+  uint32_t hi[3];
+  bad = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    t.lo[k] = 1023u;
+    hi[k] = 0u;
+  }
+  for (uint32_t i = 0; i < kW; ++i) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      t.lo[k] = min(t.lo[k], red[i * 8 + k]);
+      hi[k] = max(hi[k], red[i * 8 + 3 + k]);
+    }
+    bad |= red[i * 8 + 6];
+  }
+  bool ok = bad == 0u;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) ok = ok && t.lo[k] >= 1u && hi[k] <= 1022u;
+  t.hx = ok ? hi[0] - t.lo[0] + 3u : 0u;
+  t.hy = ok ? hi[1] - t.lo[1] + 3u : 0u;
+  const uint32_t hz = ok ? hi[2] - t.lo[2] + 3u : 0u;
+  const uint32_t nh = t.hx * t.hy * hz;
+  ok = ok && nh <= kTileHaloCells;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) t.lo[k] -= 1u;  // the halo's first cell
+  // per-thread runs of halo cells: counts, then an exclusive scan
+  const uint32_t per = ok ? (nh + BS - 1u) / BS : 0u;
+  const uint32_t h0 = min(threadIdx.x * per, nh), h1 = min(h0 + per, nh);
+  auto codeOf = [&](uint32_t h) {
+    return mortonEncode(t.lo[0] + h % t.hx, t.lo[1] + (h / t.hx) % t.hy, t.lo[2] + h / (t.hx * t.hy));
+  };
+  uint32_t sum = 0;
+  for (uint32_t h = h0; h < h1; ++h) {
+    const uint32_t code = codeOf(h);
+    sum += code + 1u < c.G ? table[code + 1u] - table[code] : 0u;
+  }
+  uint32_t xs = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t yv = uint32_t(__shfl_up(int(xs), o, 64));
+    if (__lane_id() >= uint32_t(o)) xs += yv;
+  }
+  __syncthreads();  // red is reused
+  if (__lane_id() == 63) red[w] = xs;
+  __syncthreads();
+  uint32_t run = xs - sum, total = 0;
+  for (uint32_t i = 0; i < kW; ++i) {
+    if (i < w) run += red[i];
+    total += red[i];
+  }
+  ok = ok && total <= kTileCap;
+  if (ok) {
+    for (uint32_t h = h0; h < h1; ++h) {
+      const uint32_t code = codeOf(h);
+      start[h] = run;
+      if (code + 1u < c.G) {
+        const uint32_t s0 = table[code], s1 = table[code + 1u];
+        for (uint32_t b = s0; b < s1; ++b) rec[run + (b - s0)] = src[b];
+        run += s1 - s0;
+      }
+    }
+    if (threadIdx.x == BS - 1) start[nh] = total;
+  }
+  __syncthreads();
+  t.ok = ok;
+  return t;
+}
+
+// Particle a's accepted neighbours from the staged halo (t.ok), in the
+// reference order: z rows outer, then y rows, each row the x-run of three
+// consecutive halo cells.
+template <typename Rel, typename Use>
+__device__ inline void walkHalo(const PbfConsts& c, const HaloTile& t, uint32_t key,
+                                const float4* rec, const uint32_t* start, Rel&& rel, Use&& use) {
+  const uint32_t bx = compact10(key) - t.lo[0] - 1u, by = compact10(key >> 1) - t.lo[1] - 1u,
+                 bz = compact10(key >> 2) - t.lo[2] - 1u;  // halo index of the -1 neighbours
+  for (uint32_t dz = 0; dz < 3; ++dz)
+    for (uint32_t dy = 0; dy < 3; ++dy) {
+      const uint32_t row = ((bz + dz) * t.hy + (by + dy)) * t.hx + bx;
+      const uint32_t e = start[row + 3];
+      for (uint32_t b = start[row]; b < e; ++b) {
+        const float4 pb = rec[b];
+        float dx, dy2, dz2;
+        rel(pb, dx, dy2, dz2);
+        const float d2 = dist2(dx, dy2, dz2);
+        if (d2 <= c.rmax2) use(b, pb, dx, dy2, dz2, d2);
+      }
+    }
+}
+
 // Visits the accepted neighbours (r <= h) of particle a at position pa, in
 // the reference's order: through its list when usable, else the 27 cells.
 // load(b) returns the candidate's float4 (xyz = p*). The list is walked in
@@ -494,7 +648,7 @@ __device__ inline void forEachAcceptedSplit(const PbfConsts& c, const Lists& L, 
 // PACKED (LF_FLAG_PACKED16): neighbours' positions from the fixed-point
 // records Q; the pass copies each record to QL and lambda to LAM for the
 // delta pass (as PL for the FP32 gathers).
-template <int BS, bool KAHAN, bool FAST, bool PACKED>
+template <int BS, bool KAHAN, bool FAST, bool PACKED, bool TILES = false>
 __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restrict__ keys,
                                                const uint32_t* __restrict__ table,
                                                const uint8_t* __restrict__ T,
@@ -503,8 +657,14 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
                                                Lists L, int fresh, int passIdx, int rebuild,
                                                int checkFar, const uint2* __restrict__ Q,
                                                uint2* __restrict__ QL, float* __restrict__ LAM) {
-  __shared__ uint32_t win[2 * kNbrGroup * BS];  // list rebuilds (forEachAccepted)
+  __shared__ uint32_t win[TILES ? 1 : 2 * kNbrGroup * BS];  // list rebuilds (forEachAccepted)
+  __shared__ float4 haloRec[TILES ? kTileCap : 1];
+  __shared__ uint32_t haloStart[TILES ? kTileHaloCells + 1 : 1];
+  __shared__ uint32_t haloRed[TILES ? 8 * (BS / 64) : 1];
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    HaloTile tile{};
+    if constexpr (TILES)
+      tile = stageHalo<BS>(c, ch, keys, table + c.keyBase, P, haloRec, haloStart, haloRed);
     const uint32_t a = ch.begin + threadIdx.x;
     if (a >= ch.end) return;
     const float4 pa = P[a];
@@ -523,8 +683,14 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
     auto term = [&](uint32_t, const auto&, float dx, float dy, float dz, float d2) {
       lambdaTerm<KAHAN, FAST>(c, pa.w, dx, dy, dz, d2, A);
     };
-    uint32_t* const ring = rebuild ? win : nullptr;
-    if (PACKED)
+    uint32_t* const ring = (rebuild && !TILES) ? win : nullptr;
+    if constexpr (TILES) {
+      if (tile.ok)
+        walkHalo(c, tile, keys[a], haloRec, haloStart, RelF32{pa}, term);
+      else
+        walkCells(c, keys[a], table + c.keyBase, [&](uint32_t b) { return at(P, b); }, RelF32{pa},
+                  term);
+    } else if (PACKED)
       forEachAccepted<BS, kBlockLambda, true>(
           c, L, 1, fresh != 0, passIdx, a, pa, keys[a], table + c.keyBase,
           [&](uint32_t b) { return at(Q, b); }, RelQ{unpackPos(qa), c.qUnit}, term, ring,
@@ -571,7 +737,7 @@ __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float d
 // FINAL (the last pass of a step whose passes all run): also does
 // k_finalise's work for the particle, from the same values in the same
 // order, so the separate finalise launch and its pass over the state go.
-template <int BS, bool FAST, bool FINAL, bool PACKED>
+template <int BS, bool FAST, bool FINAL, bool PACKED, bool TILES = false>
 __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict__ keys,
                                               const uint32_t* __restrict__ table,
                                               const uint8_t* __restrict__ T,
@@ -584,7 +750,13 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
                                               const float* __restrict__ LAM,
                                               uint2* __restrict__ Qout, int passIdx) {
   __shared__ float waveMax[BS / 64], waveMove[BS / 64];
+  __shared__ float4 haloRec[TILES ? kTileCap : 1];
+  __shared__ uint32_t haloStart[TILES ? kTileHaloCells + 1 : 1];
+  __shared__ uint32_t haloRed[TILES ? 8 * (BS / 64) : 1];
   forEachChunk(W, [&](const PbfConsts& c, const Chunk& ch) {
+    HaloTile tile{};
+    if constexpr (TILES)
+      tile = stageHalo<BS>(c, ch, keys, table + c.keyBase, PL, haloRec, haloStart, haloRed);
     const uint32_t a = ch.begin + threadIdx.x;
     const bool valid = a < ch.end;
     float disp = 0.0f;  // |p* - P0|, since the step's list build
@@ -600,7 +772,16 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
         const float la = pa.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         auto noRebuild = [](uint32_t) { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); };
-        if (PACKED)
+        auto dterm = [&](uint32_t, const float4& pb, float dx, float dy, float dz, float d2) {
+          deltaTerm<FAST>(c, la, pb.w, dx, dy, dz, d2, ax, ay, az);
+        };
+        if constexpr (TILES) {
+          if (tile.ok)
+            walkHalo(c, tile, keys[a], haloRec, haloStart, RelF32{pa}, dterm);
+          else
+            walkCells(c, keys[a], table + c.keyBase, [&](uint32_t b) { return at(PL, b); },
+                      RelF32{pa}, dterm);
+        } else if (PACKED)
           forEachAccepted<BS, kBlockDelta, false>(
               c, L, 2, fresh != 0, passIdx, a, pa, keys[a], table + c.keyBase,
               [&](uint32_t b) { return QRec{at(QL, b), at(LAM, b)}; },
@@ -899,6 +1080,16 @@ int pbfLambda(Ctx* c, bool skipDone) {
     return withFlag(flags & LF_FLAG_KAHAN, [&](auto kahan) {
       return withFlag(flags & LF_FLAG_FAST_MATH, [&](auto fast) {
         return withFlag(packed, [&](auto pk) {
+#if LF_PAIR_TILES
+          if constexpr (!decltype(kahan)::value && !decltype(fast)::value && !decltype(pk)::value)
+            return launch(c, name, k_lambda<decltype(bs)::value, false, false, false, true>,
+                          gridOf(c, W), dim3(decltype(bs)::value), 0, W,
+                          (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
+                          (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p,
+                          c->rho.p, listsOf(c), fresh, c->passIdx, 0,
+                          int(c->n > size_t(kNbrBias) + 1), (const uint2*)c->Q.p, c->QL.p,
+                          c->LAM.p);
+#endif
           return launch(c, name, k_lambda<decltype(bs)::value, decltype(kahan)::value,
                                           decltype(fast)::value, decltype(pk)::value>,
                         gridOf(c, W), dim3(decltype(bs)::value), 0, W,
@@ -928,6 +1119,18 @@ int pbfDelta(Ctx* c, bool skipDone, bool final) {
     return withFlag(fastMath, [&](auto fast) {
       return withFlag(final, [&](auto fin) {
         return withFlag(packed, [&](auto pk) {
+#if LF_PAIR_TILES
+          if constexpr (!decltype(fast)::value && !decltype(pk)::value)
+            return launch(c, name,
+                          k_delta<decltype(bs)::value, false, decltype(fin)::value, false, true>,
+                          gridOf(c, W), dim3(decltype(bs)::value), 0, W,
+                          (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
+                          (const uint8_t*)c->T[c->cur].p, (const float4*)c->PL.p, c->X[c->cur].p,
+                          listsOf(c), int(c->listFresh), c->P.p, c->V[c->cur].p, c->colliderGrid,
+                          (const uint32_t*)c->diffused.p,
+                          int((c->consts.flags & LF_FLAG_DIFFUSE) != 0), (const uint2*)c->QL.p,
+                          (const float*)c->LAM.p, c->Q.p, c->passIdx);
+#endif
           return launch(c, name,
                         k_delta<decltype(bs)::value, decltype(fast)::value, decltype(fin)::value,
                                 decltype(pk)::value>,
