@@ -269,6 +269,18 @@ def cpu_baseline(scene, mc, budget_s=20.0):
     return scene.n * steps / t, steps, threads
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 MIXED_FLAGS = None  # set in main: FAST_MATH | KAHAN
 
 
@@ -598,6 +610,18 @@ def main():
         out[key] = {"flags": tag, "value": n * args.steps / mixed_s,
                     "ms_per_step": 1000.0 * mixed_s / args.steps}
         log("%s: %.3f ms/step" % (key, 1000.0 * mixed_s / args.steps))
+    # the same plain FP32 steps without the marching cubes
+    ctx.upload(scene.position, scene.velocity, scene.mass, scene.type, scene.colour)
+    for _ in range(args.warmup):
+        ctx.step(p, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(p, args.steps)
+    ctx.sync()
+    nomc_s = time.perf_counter() - t0
+    out["no_mc_1m"] = {"value": n * args.steps / nomc_s, "ms_per_step": 1000.0 * nomc_s / args.steps,
+                       "note": "plain FP32, the same steps without the marching cubes"}
+    log("no_mc_1m: %.3f ms/step" % (1000.0 * nomc_s / args.steps))
     if args.batch_scenes > 0:
         out["batched"] = bench_batched(args.batch_scenes, max(20, args.steps), args.warmup,
                                        log=log)
@@ -613,7 +637,8 @@ def main():
         out["cpu_baseline"] = {
             "value": cv, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": "first %d step(s) of the same %d-particle scene incl. 256^3 marching "
-                      "cubes, FP32 CPU restatement, OpenMP" % (csteps, n)}
+                      "cubes, FP32 CPU restatement, OpenMP" % (csteps, n),
+            "cpu": cpu_model(), "host_threads": os.cpu_count()}
         out["gpu_over_cpu"] = value / cv
     print(json.dumps(out), flush=True)
 
