@@ -292,21 +292,38 @@ __device__ inline void lambdaTerm(const PbfConsts& c, float ma, float dx, float 
 }
 
 
-// Entries i .. i+BLK-1 of particle a's list as particle indices: two 12-byte
-// loads per group of 8, unpacked, b = a + entry - 2^23.
+// Block i of particle a's list (entries i .. i + BLK - 1, groups of 8) in two
+// halves: the packed words (two 12-byte loads per group), and their entries
+// as particle indices, b = a + entry - 2^23. A walk loads the
+// next block's words while the current block's gathers are in flight, so a
+// block costs one memory latency instead of two (the list load, then the
+// gathers it addresses).
+constexpr int kListWords = 6;  // words per group of 8 entries
 template <int BLK>
-__device__ inline void readListBlock(const Lists& L, uint32_t a, uint32_t i, uint32_t b[BLK]) {
-  static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
+__device__ inline void loadListWords(const Lists& L, uint32_t a, uint32_t i,
+                                     uint32_t w[BLK / kNbrGroup * kListWords]) {
   const char* const list0 = reinterpret_cast<const char*>(L.nbr) + nbrGroupByte(a, 0);
-  const uint32_t base = a - kNbrBias;
 #pragma unroll
   for (int g = 0; g < BLK / int(kNbrGroup); ++g) {
     const uint3* q =
         reinterpret_cast<const uint3*>(list0 + (i / kNbrGroup + g) * kNbrWaveGroupBytes);
     const uint3 lo = q[0], hi = q[1];
-    const uint32_t w[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+    w[g * kListWords + 0] = lo.x;
+    w[g * kListWords + 1] = lo.y;
+    w[g * kListWords + 2] = lo.z;
+    w[g * kListWords + 3] = hi.x;
+    w[g * kListWords + 4] = hi.y;
+    w[g * kListWords + 5] = hi.z;
+  }
+}
+template <int BLK>
+__device__ inline void decodeListWords(uint32_t a, const uint32_t w[BLK / kNbrGroup * kListWords],
+                                       uint32_t b[BLK]) {
+  const uint32_t base = a - kNbrBias;
+#pragma unroll
+  for (int g = 0; g < BLK / int(kNbrGroup); ++g) {
     uint32_t e[kNbrGroup];
-    unpackGroup(w, e);
+    unpackGroup(w + g * kListWords, e);
 #pragma unroll
     for (int j = 0; j < int(kNbrGroup); ++j) b[g * kNbrGroup + j] = base + e[j];
   }
@@ -573,12 +590,16 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
   if (usable) {
     static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
     static_assert(kListPad % BLK == 0, "the build pads lists to whole blocks");
+    constexpr int kW = BLK / int(kNbrGroup) * kListWords;
+    uint32_t w[kW];
+    if (cnt > 0) loadListWords<BLK>(L, a, 0, w);
     for (uint32_t i = 0; i < cnt; i += BLK) {
       uint32_t b[BLK];
-      readListBlock<BLK>(L, a, i, b);  // entries past cnt hold a
+      decodeListWords<BLK>(a, w, b);  // entries past cnt hold a
       decltype(load(0u)) p[BLK];
 #pragma unroll
       for (int k = 0; k < BLK; ++k) p[k] = load(b[k]);
+      if (i + BLK < cnt) loadListWords<BLK>(L, a, i + BLK, w);  // the next block's entries
 #pragma unroll
       for (int k = 0; k < BLK; ++k) {
         float dx, dy, dz;
@@ -615,12 +636,16 @@ __device__ inline void forEachAcceptedSplit(const PbfConsts& c, const Lists& L, 
   if (usable) {
     static_assert(BLK % kNbrGroup == 0, "blocks of whole list groups");
     static_assert(kListPad % BLK == 0, "the build pads lists to whole blocks");
+    constexpr int kW = BLK / int(kNbrGroup) * kListWords;
+    uint32_t w[kW];
+    if (cnt > 0) loadListWords<BLK>(L, a, 0, w);
     for (uint32_t i = 0; i < cnt; i += BLK) {
       uint32_t b[BLK];
-      readListBlock<BLK>(L, a, i, b);  // entries past cnt hold a
+      decodeListWords<BLK>(a, w, b);  // entries past cnt hold a
       float4 p[BLK];
 #pragma unroll
       for (int k = 0; k < BLK; ++k) p[k] = at(P, b[k]);
+      if (i + BLK < cnt) loadListWords<BLK>(L, a, i + BLK, w);  // the next block's entries
       float d2[BLK];
       bool in[BLK];
       float4 v[BLK];
