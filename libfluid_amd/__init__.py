@@ -311,6 +311,10 @@ class Context:
 
     def exec_config(self, threads=256, persistent=False, skin=0.15, particles_per_thread=1,
                     mc_tile=0):
+        """Execution configuration (results do not change): workgroup size
+        (128, 256, 512), the persistent work queue, the neighbour-list skin as
+        a fraction of h, chunks per workgroup (1, 2, 4) and the density-field
+        tile (0: 8x8x16 samples, 1: 4x16x16, 2: 8x8x8, 3: 8x8x4)."""
         e = _abi.ExecParams(threads, 1 if persistent else 0, skin, particles_per_thread, mc_tile)
         self._check(self._lib.lf_exec_configure(self._h, ctypes.byref(e)))
 
@@ -356,6 +360,17 @@ class Context:
         self.mc_classify_count(m)
         self.mc_emit(None, m)
         return self.mc_triangles()
+
+    def mesh_static(self, d_field, samples, origin, step, scale, isolevel, d_tri_pos, tri_cap,
+                    d_ntri):
+        """lf_mc_mesh_static: marching cubes of a field in device memory into
+        device buffers (addresses as integers), allocating nothing; enqueued on
+        the context's stream (sync() waits)."""
+        S = (ctypes.c_uint32 * 3)(*samples)
+        o = (ctypes.c_float * 3)(*origin)
+        st = (ctypes.c_float * 3)(*step)
+        self._check(self._lib.lf_mc_mesh_static(self._h, S, o, st, scale, isolevel, d_field,
+                                                 d_tri_pos if tri_cap else None, tri_cap, d_ntri))
 
     def mc_count(self):
         n = ctypes.c_size_t()

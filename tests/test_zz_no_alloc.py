@@ -39,12 +39,9 @@ def _static(c, field, origin, step, scale, iso, cap):
     d_tri = torch.full((max(cap, 1) * 9,), float("nan"), dtype=torch.float32, device="cuda:0")
     d_n = torch.full((1,), -1, dtype=torch.int32, device="cuda:0")
     torch.cuda.synchronize()
-    S = (ctypes.c_uint32 * 3)(*field.shape)
-    o = (ctypes.c_float * 3)(*origin)
-    st = (ctypes.c_float * 3)(*step)
     print("lf_mc_mesh_static samples %s cap %d" % (field.shape, cap), flush=True)
-    c._check(c._lib.lf_mc_mesh_static(c._h, S, o, st, scale, iso, d_field.data_ptr(),
-                                      d_tri.data_ptr() if cap else None, cap, d_n.data_ptr()))
+    c.mesh_static(d_field.data_ptr(), field.shape, origin, step, scale, iso, d_tri.data_ptr(),
+                  cap, d_n.data_ptr())
     c.sync()
     n = int(d_n.cpu().numpy().view(np.uint32)[0])
     tris = d_tri.cpu().numpy()[:min(n, cap) * 9].reshape(-1, 3, 3)
