@@ -62,6 +62,15 @@ int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string
     const double r = double(c.h) + skin;
     c.listR2 = float(r * r * (1.0 + 1e-5));
     c.skinSafe = float(skin * (1.0 - 1e-5));
+    // lists rebuilt by a lambda pass (where particles moved more than the
+    // skin) take a wider skin, 3x up to 0.45 h (about 100 entries at rest,
+    // within the 128-entry capacity), so later passes rebuild them less often:
+    // at 1M over steps 5-25, 1.659 -> 1.607 ms/step (1.5x 1.662, 2x 1.638,
+    // 2.5x 1.607, 3.5x 1.612, 4x 1.909: lists overflow)
+    const double skinR = std::max(skin, std::min(3.0 * skin, 0.45 * double(c.h)));
+    const double rR = double(c.h) + skinR;
+    c.listR2R = float(rR * rR * (1.0 + 1e-5));
+    c.skinSafeR = float(skinR * (1.0 - 1e-5));
   }
   c.flags = p.flags;
   c.densityTol = p.density_tol > 0.0f ? p.density_tol : 0.0f;
@@ -72,6 +81,8 @@ int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string
     // reuse test keeps two quanta of margin (sqrt(3) would do)
     c.skinSafe -= 2.0f * c.qUnit;
     if (!(c.skinSafe > 0.0f)) c.skinSafe = 0.0f;
+    c.skinSafeR -= 2.0f * c.qUnit;
+    if (!(c.skinSafeR > 0.0f)) c.skinSafeR = 0.0f;
   }
   c.wdq = hostW6(c, p.corr_dq * c.h);
   c.wdqRecip = 1.0f / c.wdq;

@@ -46,6 +46,8 @@ struct PbfConsts {
       vd, xsph, xsphRho;  // xsphRho = xsph / rho0
   float listR2;      // neighbour-list radius^2, (h + skin)^2 with rounding margin
   float skinSafe;    // skin less a rounding margin
+  float listR2R;     // the same for lists rebuilt by a lambda pass (a wider skin)
+  float skinSafeR;
   float minExt[3];   // minBound/scale - 2h (oclsph.hpp:359)
   float minB[3], maxB[3];
   float grav[3];

@@ -95,7 +95,7 @@ __device__ inline bool listUsable(const PbfConsts& c, const Lists& L, uint32_t a
     const float4 r = L.R[a];
     float s = __builtin_sqrtf(dist2(x - r.x, y - r.y, z - r.z));
     for (int j = int(ep); j < passIdx && j < kPassSlots; ++j) s += __uint_as_float(L.maxDisp[4 + j]);
-    return passIdx < kPassSlots && s < c.skinSafe;
+    return passIdx < kPassSlots && s < c.skinSafeR;
   }
   const float4 p0 = L.P0[a];
   const float da = __builtin_sqrtf(dist2(x - p0.x, y - p0.y, z - p0.z));
@@ -129,7 +129,7 @@ __device__ inline bool listUsableLocal(const PbfConsts& c, const Lists& L, uint3
     m = max(m, code + 1u < c.G ? d[i] : 0u);
   }
   const float mb = __uint_as_float(m);
-  return da + (ep > 1u ? mb + mb : mb) < c.skinSafe;
+  return ep > 1u ? da + (mb + mb) < c.skinSafeR : da + mb < c.skinSafe;
 }
 
 // base[i] addressed as base + a 32-bit byte offset (the base stays in scalar
@@ -169,8 +169,8 @@ __device__ inline const float4& posOf(const PosQ& r) { return r.p; }
 // count (kMaxNbr + 1: overflowed, or a candidate 2^23 positions away) goes
 // to L.count[a].
 template <int BS, int BB = kBuildBlock, typename Load, typename Visit>
-__device__ inline void buildList(const PbfConsts& c, const Lists& L, uint32_t* __restrict__ win,
-                                 uint32_t a, float4 pa, uint32_t key,
+__device__ inline void buildList(const PbfConsts& c, float listR2, const Lists& L,
+                                 uint32_t* __restrict__ win, uint32_t a, float4 pa, uint32_t key,
                                  const uint32_t* __restrict__ table, int checkFar, Load&& load,
                                  Visit&& visit) {
   constexpr uint32_t kRing = 2 * kNbrGroup;
@@ -209,7 +209,7 @@ __device__ inline void buildList(const PbfConsts& c, const Lists& L, uint32_t* _
         // cell): no branch per candidate
         const uint32_t accept =
             (b < e ? 1u : 0u) &
-            (dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z) <= c.listR2 ? 1u : 0u);
+            (dist2(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z) <= listR2 ? 1u : 0u);
         // written whether accepted or not: slot cnt % 16 is the next free
         // one, so a rejected candidate is overwritten by the next accepted
         win[(cnt % kRing) * BS + threadIdx.x] = b + bias;
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(BS) void k_build_list(Work W, const uint32_t* __res
       L.count[a] = 0;
       return;
     }
-    buildList<BS>(c, L, win, a, L.P0[a], keys[a], table + c.keyBase, checkFar,
+    buildList<BS>(c, c.listR2, L, win, a, L.P0[a], keys[a], table + c.keyBase, checkFar,
                   [&](uint32_t b) { return at(L.P0, b); }, [](uint32_t, const float4&, bool) {});
   });
 }
@@ -575,7 +575,7 @@ __device__ inline void forEachAccepted(const PbfConsts& c, const Lists& L, int p
   if (fell) usable = false;
   if constexpr (REBUILD) {
     if (!usable && win && passIdx < kPassSlots) {
-      buildList<BS, kRebuildBlock>(c, L, win, a, pa, key, table, checkFar, loadPos,
+      buildList<BS, kRebuildBlock>(c, c.listR2R, L, win, a, pa, key, table, checkFar, loadPos,
                     [&](uint32_t b, const auto& r, bool valid) {
                       float dx, dy, dz;
                       rel(r, dx, dy, dz);
