@@ -83,11 +83,12 @@ KERNEL_SYMBOL = {
     "predict_count": "k_predict_count", "bucket_reduce": "k_bucket_reduce",
     "bucket_scan": "k_bucket_scan", "bucket_scatter": "k_bucket_scatter",
     "sort_big": "k_sort_big", "reorder": "k_reorder", "pbf_build_list": "k_build_list",
-    # k_lambda<threads, KAHAN, FAST_MATH, PACKED16>, k_delta<threads, FAST_MATH, FINAL, PACKED16>
-    "pbf_lambda": r"k_lambda<\d+, false, false, false>",
-    "pbf_lambda_kahan": r"k_lambda<\d+, true, false, false>",
-    "pbf_delta": r"k_delta<\d+, false, false, false>",
-    "pbf_delta_final": r"k_delta<\d+, false, true, false>",
+    # k_lambda<threads, KAHAN, arith, PACKED16, tiles>, k_delta<threads, arith, FINAL, PACKED16,
+    # tiles>; arith 0: IEEE sequences, 1: their shortened correctly rounded forms, 2: FAST_MATH
+    "pbf_lambda": r"k_lambda<\d+, false, [01], false, false>",
+    "pbf_lambda_kahan": r"k_lambda<\d+, true, [01], false, false>",
+    "pbf_delta": r"k_delta<\d+, [01], false, false, false>",
+    "pbf_delta_final": r"k_delta<\d+, [01], true, false, false>",
     "pbf_finalise": "k_finalise", "pbf_xsph": "k_xsph",
     "pbf_diffuse": "k_diffuse", "mc_density_field": "k_mc_density", "mc_classify_count": "k_mc_count",
     "mc_scan": "k_mc_scan", "mc_emit": "k_mc_emit", "copy_probe": "k_copy_probe",

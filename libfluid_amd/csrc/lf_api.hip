@@ -43,6 +43,7 @@ int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string
   }
   const float pi = 3.14159265358979323846f;
   c.h = p.h;
+  c.rnDiv = (c.h >= 1e-4f && c.h <= 1e4f) ? 1u : 0u;
   c.hh = c.h * c.h;
   const float hhh = c.hh * c.h;
   c.poly6 = 315.0f / ((64.0f * pi) * ((hhh * hhh) * hhh));
@@ -167,7 +168,6 @@ int prepareScenes(Ctx* c) {
   if (!c->batch) {
     PbfConsts k = c->consts;
     k.keyBase = 0;
-    k.pad0 = 0;
     consts.push_back(k);
     begin = {0u, uint32_t(c->n)};
     total = uint64_t(k.G) + 1u;
@@ -179,7 +179,6 @@ int prepareScenes(Ctx* c) {
       const int rc = foldConsts(c->sceneParams[s], c->skinFrac, consts[s], err);
       if (rc) return fail(c, rc, "scene " + std::to_string(s) + ": " + err);
       consts[s].keyBase = uint32_t(total);
-      consts[s].pad0 = 0;
       total += uint64_t(consts[s].G) + 1u;
     }
     begin = c->sceneBegin;
@@ -198,6 +197,8 @@ int prepareScenes(Ctx* c) {
   c->tableTotal = c->batch ? uint32_t(total) : consts[0].G;
   c->converge = false;
   for (const PbfConsts& k : consts) c->converge = c->converge || k.densityTol > 0.0f;
+  c->pairRn = true;
+  for (const PbfConsts& k : consts) c->pairRn = c->pairRn && k.rnDiv != 0u;
   if (!c->batch) c->sceneBegin = begin;
   if (sameConsts && sameChunks && c->dConsts.p && c->chunks.p) return LF_OK;
   if (c->capturing)

@@ -20,7 +20,8 @@
 //  * the library is compiled with -ffp-contract=off; each fused multiply-add
 //    is an explicit __builtin_fmaf;
 //  * sqrt and division are IEEE correctly rounded (hipcc's default
-//    -fhip-fp32-correctly-rounded-divide-sqrt);
+//    -fhip-fp32-correctly-rounded-divide-sqrt; the pair terms use shortened
+//    forms of the same sequences where they give the same bits, kArithRn);
 //  * d2 = fma(dz,dz, fma(dy,dy, dx*dx)) with d = p_a - p_b;
 //  * W6(r) = poly6 * t*t*t, t = fma(-r, r, h*h), for r <= h
 //    gradW  = d * (spiky * ((h-r)*(h-r) / r)) for 1e-8 <= r <= h
@@ -59,7 +60,7 @@ struct PbfConsts {
   // no cell and share the scene's last bucket, so scenes sort apart and each
   // keeps the reference's order of its cells.
   uint32_t keyBase;
-  uint32_t pad0;
+  uint32_t rnDiv;    // 1 when 1e-4 <= h <= 1e4 (the kArithRn forms are exact)
   float densityTol;  // > 0: skip the remaining passes once the density error is below it
   float qUnit;       // LF_FLAG_PACKED16: one fixed-point quantum, h / 65536 (solver units)
   double qInv;       // 65536 / h
@@ -290,6 +291,48 @@ __device__ inline float poly6W(const PbfConsts& c, float r) {
   const float t = __builtin_fmaf(-r, r, c.hh);
   return c.poly6 * ((t * t) * t);
 }
+// Arithmetic of the pair terms' square root and division (a template
+// argument of the lambda and delta passes):
+//  * kArithIeee: hipCC's IEEE sequences (sqrtf, FP32 division), for any h;
+//  * kArithRn: sqrtRn and divRn below, the same bits in fewer instructions
+//    when 1e-4 <= h <= 1e4 (PbfConsts::rnDiv);
+//  * kArithHw: the hardware approximations (LF_FLAG_FAST_MATH).
+enum Arith : int { kArithIeee = 0, kArithRn = 1, kArithHw = 2 };
+
+// The kArithRn forms are the compiler's own sequences with the steps that only
+// act near the ends of the FP32 range left out:
+//  * sqrtRn: for x >= 2^-96 the compiler's sqrtf takes the hardware estimate
+//    s (within 1 ulp) and returns s - 1 ulp, s or s + 1 ulp by the signs of
+//    the residuals x - s' * s; sqrtRn does exactly that. Below 2^-96 sqrtf
+//    first scales x by 2^32 and passes 0 through; sqrtRn does not, and returns
+//    some r <= 2^-47 where sqrtf gives one below 2^-48. For a pair term the
+//    two agree: r < 1e-8 gives a zero spiky term either way, and r * r is far
+//    below half an ulp of h * h >= 1e-8, so fma(-r, r, h * h) = h * h.
+//  * divRn: the compiler's FP32 division refines the hardware reciprocal y
+//    of b once (y += y * (1 - b y)), takes q = a y and corrects it twice
+//    (q += y * (a - b q)). Its scale steps (v_div_scale, v_div_fmas) act only
+//    when b or 1/b is denormal, a is below 2^-103, or a/b is denormal or 2^96
+//    beyond in exponent, and its fix-up step only settles zeros, infinities
+//    and NaNs. For the spiky term's a = (h - r)^2, b = r, 1e-8 <= r <= h and
+//    1e-4 <= h <= 1e4, a is 0 or at least (h 2^-25)^2 and none of these
+//    occurs; a = 0 gives +0 either way.
+__device__ inline float sqrtRn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+  const float su = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rd = __builtin_fmaf(-sd, s, x);
+  const float ru = __builtin_fmaf(-su, s, x);
+  const float t = rd <= 0.0f ? sd : s;
+  return ru > 0.0f ? su : t;
+}
+__device__ inline float divRn(float a, float b) {
+  float y = __builtin_amdgcn_rcpf(b);
+  y = __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+  float q = a * y;
+  q = __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+  return __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+}
+
 // caller guarantees r <= h
 __device__ inline float spikyScale(const PbfConsts& c, float r) {
   if (!(r >= 1e-8f)) return 0.0f;
@@ -297,15 +340,14 @@ __device__ inline float spikyScale(const PbfConsts& c, float r) {
   return c.spiky * ((q * q) / r);
 }
 
-// sqrt and x / r: correctly rounded, or the hardware approximations (FAST).
-template <bool FAST> __device__ inline float sqrtM(float x) {
-  return FAST ? __builtin_amdgcn_sqrtf(x) : __builtin_sqrtf(x);
+template <int AR> __device__ inline float sqrtM(float x) {
+  return AR == kArithHw ? __builtin_amdgcn_sqrtf(x) : AR == kArithRn ? sqrtRn(x) : __builtin_sqrtf(x);
 }
-template <bool FAST> __device__ inline float spikyScaleM(const PbfConsts& c, float r) {
-  if (!FAST) return spikyScale(c, r);
+template <int AR> __device__ inline float spikyScaleM(const PbfConsts& c, float r) {
+  if (AR == kArithIeee) return spikyScale(c, r);
   if (!(r >= 1e-8f)) return 0.0f;
   const float q = c.h - r;
-  return c.spiky * ((q * q) * __builtin_amdgcn_rcpf(r));
+  return c.spiky * (AR == kArithRn ? divRn(q * q, r) : (q * q) * __builtin_amdgcn_rcpf(r));
 }
 
 // 27-cell visit in the reference order (oclsph_kernel.h:150-192): x fastest,
@@ -644,6 +686,7 @@ struct Ctx {
   DevBuf<uint32_t> sceneDone;              // per scene: converged in this step
   DevBuf<uint32_t> scenePasses;            // per scene: lambda passes run in the last step
   bool converge = false;                   // some scene has density_tol > 0
+  bool pairRn = true;                      // every scene has rnDiv: kArithRn pair terms
   float mcScale = 1.0f;                    // world scale of the marching-cubes output
   DevBuf<uint32_t> localKey;               // scene-local Morton key per sorted particle
   int mcScene = 0;

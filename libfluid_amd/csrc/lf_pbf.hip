@@ -272,12 +272,12 @@ __device__ inline void neumaier(float& sum, float& comp, float term) {
 
 // One accepted pair of sph_lambda (oclsph_kernel.h:324-326): gradient sum
 // g += gradW / rho0 and density rho += m_a W6(r).
-template <bool KAHAN, bool FAST>
+template <bool KAHAN, int AR>
 __device__ inline void lambdaTerm(const PbfConsts& c, float ma, float dx, float dy, float dz,
                                   float d2, LambdaSums& A) {
-  const float r = sqrtM<FAST>(d2);
+  const float r = sqrtM<AR>(d2);
   const float w = poly6W(c, r);
-  const float sc = spikyScaleM<FAST>(c, r);
+  const float sc = spikyScaleM<AR>(c, r);
   if (!KAHAN) {
     A.gx = __builtin_fmaf(dx * sc, c.rhoRecip, A.gx);
     A.gy = __builtin_fmaf(dy * sc, c.rhoRecip, A.gy);
@@ -404,7 +404,7 @@ struct HaloTile {
   bool ok;
 };
 
-__device__ inline uint32_t compact10(uint32_t v) {  // inverse of spread10
+[[maybe_unused]] __device__ inline uint32_t compact10(uint32_t v) {  // inverse of spread10
   v &= 0x09249249u;
   v = (v | (v >> 2)) & 0x030C30C3u;
   v = (v | (v >> 4)) & 0x0300F00Fu;
@@ -673,7 +673,7 @@ __device__ inline void forEachAcceptedSplit(const PbfConsts& c, const Lists& L, 
 // PACKED (LF_FLAG_PACKED16): neighbours' positions from the fixed-point
 // records Q; the pass copies each record to QL and lambda to LAM for the
 // delta pass (as PL for the FP32 gathers).
-template <int BS, bool KAHAN, bool FAST, bool PACKED, bool TILES = false>
+template <int BS, bool KAHAN, int AR, bool PACKED, bool TILES = false>
 __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restrict__ keys,
                                                const uint32_t* __restrict__ table,
                                                const uint8_t* __restrict__ T,
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
     }
     LambdaSums A;
     auto term = [&](uint32_t, const auto&, float dx, float dy, float dz, float d2) {
-      lambdaTerm<KAHAN, FAST>(c, pa.w, dx, dy, dz, d2, A);
+      lambdaTerm<KAHAN, AR>(c, pa.w, dx, dy, dz, d2, A);
     };
     uint32_t* const ring = (rebuild && !TILES) ? win : nullptr;
     if constexpr (TILES) {
@@ -741,14 +741,14 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
 }
 
 // One accepted pair of sph_delta (oclsph_kernel.h:355-358).
-template <bool FAST>
+template <int AR>
 __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float dx, float dy,
                                  float dz, float d2, float& ax, float& ay, float& az) {
-  const float r = sqrtM<FAST>(d2);
+  const float r = sqrtM<AR>(d2);
   const float q = poly6W(c, r) * c.wdqRecip;
   const float corr = -c.corrK * ((q * q) * (q * q));
   const float factor = ((la + lb) + corr) * c.rhoRecip;
-  const float sc = spikyScaleM<FAST>(c, r);
+  const float sc = spikyScaleM<AR>(c, r);
   ax = __builtin_fmaf(dx * sc, factor, ax);
   ay = __builtin_fmaf(dy * sc, factor, ay);
   az = __builtin_fmaf(dz * sc, factor, az);
@@ -762,7 +762,7 @@ __device__ inline void deltaTerm(const PbfConsts& c, float la, float lb, float d
 // FINAL (the last pass of a step whose passes all run): also does
 // k_finalise's work for the particle, from the same values in the same
 // order, so the separate finalise launch and its pass over the state go.
-template <int BS, bool FAST, bool FINAL, bool PACKED, bool TILES = false>
+template <int BS, int AR, bool FINAL, bool PACKED, bool TILES = false>
 __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict__ keys,
                                               const uint32_t* __restrict__ table,
                                               const uint8_t* __restrict__ T,
@@ -798,7 +798,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         auto noRebuild = [](uint32_t) { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); };
         auto dterm = [&](uint32_t, const float4& pb, float dx, float dy, float dz, float d2) {
-          deltaTerm<FAST>(c, la, pb.w, dx, dy, dz, d2, ax, ay, az);
+          deltaTerm<AR>(c, la, pb.w, dx, dy, dz, d2, ax, ay, az);
         };
         if constexpr (TILES) {
           if (tile.ok)
@@ -812,7 +812,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
               [&](uint32_t b) { return QRec{at(QL, b), at(LAM, b)}; },
               RelQ{unpackPos(QL[a]), c.qUnit},
               [&](uint32_t, const QRec& r, float dx, float dy, float dz, float d2) {
-                deltaTerm<FAST>(c, la, r.lam, dx, dy, dz, d2, ax, ay, az);
+                deltaTerm<AR>(c, la, r.lam, dx, dy, dz, d2, ax, ay, az);
               },
               nullptr, 0, noRebuild);
         else
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(BS) void k_delta(Work W, const uint32_t* __restrict
               c, L, 2, fresh != 0, passIdx, a, pa, keys[a], table + c.keyBase,
               [&](uint32_t b) { return at(PL, b); }, RelF32{pa},
               [&](uint32_t, const float4& pb, float dx, float dy, float dz, float d2) {
-                deltaTerm<FAST>(c, la, pb.w, dx, dy, dz, d2, ax, ay, az);
+                deltaTerm<AR>(c, la, pb.w, dx, dy, dz, d2, ax, ay, az);
               },
               nullptr, 0, noRebuild);
         // pos = (p* + dp) * scale, mesh colliders (:364-368), clamped to the
@@ -1083,6 +1083,14 @@ template <typename F> int withBS(const Ctx* c, F&& f) {
 template <typename F> int withFlag(bool on, F&& f) {
   return on ? f(std::true_type{}) : f(std::false_type{});
 }
+// The pair terms' arithmetic: the hardware approximations under
+// LF_FLAG_FAST_MATH, else the short correctly rounded forms where every
+// scene's h allows them, else the compiler's IEEE sequences.
+template <typename F> int withArith(const Ctx* c, bool fast, F&& f) {
+  if (fast) return f(std::integral_constant<int, kArithHw>{});
+  if (c->pairRn) return f(std::integral_constant<int, kArithRn>{});
+  return f(std::integral_constant<int, kArithIeee>{});
+}
 
 // The packed records exist only when the grid was built with the flag.
 static int requirePacked(Ctx* c, bool packed) {
@@ -1103,11 +1111,12 @@ int pbfLambda(Ctx* c, bool skipDone) {
   const bool packed = (flags & LF_FLAG_PACKED16) != 0;
   return withBS(c, [&](auto bs) {
     return withFlag(flags & LF_FLAG_KAHAN, [&](auto kahan) {
-      return withFlag(flags & LF_FLAG_FAST_MATH, [&](auto fast) {
+      return withArith(c, flags & LF_FLAG_FAST_MATH, [&](auto ar) {
         return withFlag(packed, [&](auto pk) {
 #if LF_PAIR_TILES
-          if constexpr (!decltype(kahan)::value && !decltype(fast)::value && !decltype(pk)::value)
-            return launch(c, name, k_lambda<decltype(bs)::value, false, false, false, true>,
+          if constexpr (!decltype(kahan)::value && decltype(ar)::value != kArithHw &&
+                        !decltype(pk)::value)
+            return launch(c, name, k_lambda<decltype(bs)::value, false, decltype(ar)::value, false, true>,
                           gridOf(c, W), dim3(decltype(bs)::value), 0, W,
                           (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
                           (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p,
@@ -1116,7 +1125,7 @@ int pbfLambda(Ctx* c, bool skipDone) {
                           c->LAM.p);
 #endif
           return launch(c, name, k_lambda<decltype(bs)::value, decltype(kahan)::value,
-                                          decltype(fast)::value, decltype(pk)::value>,
+                                          decltype(ar)::value, decltype(pk)::value>,
                         gridOf(c, W), dim3(decltype(bs)::value), 0, W,
                         (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
                         (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p,
@@ -1141,13 +1150,14 @@ int pbfDelta(Ctx* c, bool skipDone, bool final) {
                            : (fastMath ? "pbf_delta_fast" : "pbf_delta");
   const bool packed = (c->consts.flags & LF_FLAG_PACKED16) != 0;
   const int rc = withBS(c, [&](auto bs) {
-    return withFlag(fastMath, [&](auto fast) {
+    return withArith(c, fastMath, [&](auto ar) {
       return withFlag(final, [&](auto fin) {
         return withFlag(packed, [&](auto pk) {
 #if LF_PAIR_TILES
-          if constexpr (!decltype(fast)::value && !decltype(pk)::value)
+          if constexpr (decltype(ar)::value != kArithHw && !decltype(pk)::value)
             return launch(c, name,
-                          k_delta<decltype(bs)::value, false, decltype(fin)::value, false, true>,
+                          k_delta<decltype(bs)::value, decltype(ar)::value, decltype(fin)::value,
+                                  false, true>,
                           gridOf(c, W), dim3(decltype(bs)::value), 0, W,
                           (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
                           (const uint8_t*)c->T[c->cur].p, (const float4*)c->PL.p, c->X[c->cur].p,
@@ -1157,7 +1167,7 @@ int pbfDelta(Ctx* c, bool skipDone, bool final) {
                           (const float*)c->LAM.p, c->Q.p, c->passIdx);
 #endif
           return launch(c, name,
-                        k_delta<decltype(bs)::value, decltype(fast)::value, decltype(fin)::value,
+                        k_delta<decltype(bs)::value, decltype(ar)::value, decltype(fin)::value,
                                 decltype(pk)::value>,
                         gridOf(c, W), dim3(decltype(bs)::value), 0, W,
                         (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,

@@ -135,6 +135,51 @@ def test_steps_bit_exact_8k(ctx, steps):
     _eq(g["colour"], o.get(oracle.CALLER_COLOUR), "colour")
 
 
+def test_steps_bit_exact_small_h(ctx):
+    """A solver h below 1e-4: the lambda and delta passes take the compiler's
+    full IEEE sqrt and division (their shortened forms are exact only for
+    1e-4 <= h <= 1e4). The 8k scene in solver units shrunk by f = 9e-4 (h and
+    the world scale), with gravity, rest density, cfm and the s_corr constant
+    scaled so the solver sees the same motion up to rounding (gravity ~ f,
+    rho0 ~ f^-3, cfm ~ f^-2, corr_k ~ f^2)."""
+    import dataclasses
+    sc = scenes.dam_break_8k()
+    f = 9e-4
+    sc = dataclasses.replace(sc, h=sc.h * f, scale=sc.scale / f,
+                             gravity=tuple(g * f for g in sc.gravity))
+    p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
+    assert p.h < 1e-4
+    p.rest_density = p.rest_density / f ** 3
+    p.cfm_epsilon = p.cfm_epsilon / f ** 2
+    p.corr_k = p.corr_k * f ** 2
+    o = _pair(ctx, sc, p)
+    ctx.grid_build(p)
+    o.grid_build()
+    for it in range(p.iterations):
+        ctx.lambda_pass(p)
+        o.lambda_pass()
+        lam = ctx.read(LF_ARR_LAMBDA)[:, 3]
+        _eq(lam.astype(np.float64), o.get(LF_ARR_LAMBDA), "lambda it%d" % it)
+        assert np.abs(lam).max() > 0.0
+        ctx.delta_pass(p)
+        o.delta_pass()
+        _eq(ctx.read(LF_ARR_PSTAR)[:, :3].astype(np.float64), o.get(LF_ARR_PSTAR)[:, :3],
+            "p* it%d" % it)
+    ctx.finalise(p)
+    o.finalise()
+    _eq(ctx.read(LF_ARR_POSITION)[:, :3].astype(np.float64), o.get(LF_ARR_POSITION), "x")
+    ctx.xsph(p)
+    o.xsph()
+    _eq(ctx.read(LF_ARR_VELOCITY)[:, :3].astype(np.float64), o.get(LF_ARR_VELOCITY), "v xsph")
+    # whole steps from the start
+    o = _pair(ctx, sc, p)
+    ctx.step(p, 3)
+    o.step(3)
+    g = ctx.download()
+    _eq(g["position"].astype(np.float64), o.get(oracle.CALLER_POSITION), "position")
+    _eq(g["velocity"].astype(np.float64), o.get(oracle.CALLER_VELOCITY), "velocity")
+
+
 def test_step_1m_bit_exact(ctx):
     sc = scenes.dam_break_1m()
     p = sc.params()
