@@ -343,11 +343,12 @@ __device__ inline float spikyScale(const PbfConsts& c, float r) {
 template <int AR> __device__ inline float sqrtM(float x) {
   return AR == kArithHw ? __builtin_amdgcn_sqrtf(x) : AR == kArithRn ? sqrtRn(x) : __builtin_sqrtf(x);
 }
+// (the shortened forms compute the quotient for every r and select: no branch)
 template <int AR> __device__ inline float spikyScaleM(const PbfConsts& c, float r) {
   if (AR == kArithIeee) return spikyScale(c, r);
-  if (!(r >= 1e-8f)) return 0.0f;
   const float q = c.h - r;
-  return c.spiky * (AR == kArithRn ? divRn(q * q, r) : (q * q) * __builtin_amdgcn_rcpf(r));
+  const float v = c.spiky * (AR == kArithRn ? divRn(q * q, r) : (q * q) * __builtin_amdgcn_rcpf(r));
+  return r >= 1e-8f ? v : 0.0f;
 }
 
 // 27-cell visit in the reference order (oclsph_kernel.h:150-192): x fastest,
