@@ -379,22 +379,32 @@ __global__ __launch_bounds__(BS) void k_reorder(
     if (s >= ch.end) return;
     const uint32_t k = tmpKey[s];
     const uint32_t i = min(tmpVal[s], n - 1u);  // a permutation; the clamp only bounds reads
-    const uint32_t b0 = table[k], b1 = table[k + 1u];
-    uint32_t f = s;  // buckets above kFixupMax were put in order by k_sort_big
-    if (b1 - b0 <= kFixupMax) {
-      f = b0;
-      for (uint32_t t = b0; t < b1; ++t) f += tmpVal[t] < i ? 1u : 0u;
-    }
-    f = min(f, n - 1u);
+    // the state gathers are issued before the rank search (they do not depend on it)
     const float4 x = Xin[i];
     const float4 v = Vin[i];
     const uint8_t t = Tin[i];
+    const uint32_t id = IDin[i];
+    const uint32_t b0 = table[k], b1 = table[k + 1u];
+    uint32_t f = s;  // buckets above kFixupMax were put in order by k_sort_big
+    if (b1 - b0 <= kFixupMax) {
+      // the bucket's pre-step indices 8 at a time (indices clamped into the
+      // bucket): one memory latency per 8 entries, not per entry
+      f = b0;
+      for (uint32_t u0 = b0; u0 < b1; u0 += 8u) {
+        uint32_t w[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; ++u) w[u] = tmpVal[min(u0 + u, b1 - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; ++u) f += (u0 + u < b1 && w[u] < i) ? 1u : 0u;
+      }
+    }
+    f = min(f, n - 1u);
     float px, py, pz;
     predictPstar(c, x, v, t, wells, nWells, px, py, pz);
     Xout[f] = x;
     Vout[f] = v;
     Tout[f] = t;
-    IDout[f] = IDin[i];
+    IDout[f] = id;
     P[f] = make_float4(px, py, pz, x.w);
     localKey[f] = cellKey(c, px, py, pz);
     P0[f] = make_float4(clampToBounds(c, 0, px), clampToBounds(c, 1, py), clampToBounds(c, 2, pz),
