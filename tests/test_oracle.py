@@ -249,7 +249,7 @@ def test_lambda_lattice_kat_exact():
     assert math.isclose(lam[centre], -(want / 6378 - 1) / 600, rel_tol=1e-9)
 
 
-def test_lambda_lattice_kat_survey_values():
+def test_lambda_lattice_kat_half_h_spacing():
     """s = 0.05, h = 0.1, m = 1: rho = 8078.2013, lambda = -4.4429e-4 (float32
     positions, hence the 2e-6 tolerance)."""
     pos = _lattice64(12, 0.05, 0.3)
