@@ -110,7 +110,7 @@ int foldConsts(const lf_pbf_params& p, float skinFrac, PbfConsts& c, std::string
   return LF_OK;
 }
 
-void timingBegin(Ctx* c, const char* name, PendingTiming& t) {
+void timingEvents(Ctx* c, const char* name, PendingTiming& t) {
   t.name = name;
   for (int i = 0; i < 2; ++i) {
     hipEvent_t e;
@@ -122,11 +122,14 @@ void timingBegin(Ctx* c, const char* name, PendingTiming& t) {
     }
     (i == 0 ? t.start : t.stop) = e;
   }
-  if (t.start) (void)hipEventRecord(t.start, c->launchStream ? c->launchStream : c->stream);
+  if (!(t.start && t.stop)) {  // both or none: a launch stamps what it is given
+    for (hipEvent_t e : {t.start, t.stop})
+      if (e) c->eventPool.push_back(e);
+    t.start = t.stop = nullptr;
+  }
 }
 
-void timingEnd(Ctx* c, PendingTiming& t) {
-  if (t.stop) (void)hipEventRecord(t.stop, c->launchStream ? c->launchStream : c->stream);
+void timingQueue(Ctx* c, PendingTiming& t) {
   if (t.start && t.stop) c->pending.push_back(t);
 }
 

@@ -4,6 +4,7 @@
 // root (tom91136/libfluid).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -828,9 +829,11 @@ template <typename T> void release(DevBuf<T>& b) {
   b.cap = 0;
 }
 
-// Kernel timing brackets (profiling only).
-void timingBegin(Ctx* c, const char* name, PendingTiming& t);
-void timingEnd(Ctx* c, PendingTiming& t);
+// Kernel timing (profiling only): two events of the pool, which the launch
+// itself stamps with the kernel's start and end (hipExtLaunchKernel: no
+// marker packets of their own around the dispatch), queued for reading.
+void timingEvents(Ctx* c, const char* name, PendingTiming& t);
+void timingQueue(Ctx* c, PendingTiming& t);
 
 template <typename... KArgs, typename... Args>
 int launch(Ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
@@ -838,11 +841,16 @@ int launch(Ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 b
   PendingTiming t{};
   const bool timed = c->prof && (c->profOnly.empty() || c->profOnly == name);
   const hipStream_t stream = c->launchStream ? c->launchStream : c->stream;
-  if (timed) timingBegin(c, name, t);
-  hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
+  if (timed) {
+    timingEvents(c, name, t);
+    hipExtLaunchKernelGGL(kernel, grid, block, uint32_t(shmem), stream, t.start, t.stop, 0u,
+                          args...);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hipFail(c, e, name);
-  if (timed) timingEnd(c, t);
+  if (timed) timingQueue(c, t);
   return LF_OK;
 }
 
