@@ -682,7 +682,10 @@ __global__ __launch_bounds__(BS) void k_lambda(Work W, const uint32_t* __restric
                                                Lists L, int fresh, int passIdx, int rebuild,
                                                int checkFar, const uint2* __restrict__ Q,
                                                uint2* __restrict__ QL, float* __restrict__ LAM) {
-  __shared__ uint32_t win[TILES ? 1 : 2 * kNbrGroup * BS];  // list rebuilds (forEachAccepted)
+  // the ring of list rebuilds (forEachAccepted), 2 * kNbrGroup * BS words:
+  // dynamic LDS, given only to the passes that may rebuild, so that the
+  // first pass leaves that LDS to the kernels beside it
+  extern __shared__ uint32_t win[];
   __shared__ float4 haloRec[TILES ? kTileCap : 1];
   __shared__ uint32_t haloStart[TILES ? kTileHaloCells + 1 : 1];
   __shared__ uint32_t haloRed[TILES ? 8 * (BS / 64) : 1];
@@ -1124,12 +1127,15 @@ int pbfLambda(Ctx* c, bool skipDone) {
                           int(c->n > size_t(kNbrBias) + 1), (const uint2*)c->Q.p, c->QL.p,
                           c->LAM.p);
 #endif
+          const int rebuild = int(c->passIdx > 1);
+          const size_t ring =
+              rebuild ? 2u * kNbrGroup * uint32_t(decltype(bs)::value) * sizeof(uint32_t) : 0u;
           return launch(c, name, k_lambda<decltype(bs)::value, decltype(kahan)::value,
                                           decltype(ar)::value, decltype(pk)::value>,
-                        gridOf(c, W), dim3(decltype(bs)::value), 0, W,
+                        gridOf(c, W), dim3(decltype(bs)::value), ring, W,
                         (const uint32_t*)c->localKey.p, (const uint32_t*)c->table.p,
                         (const uint8_t*)c->T[c->cur].p, (const float4*)c->P.p, c->PL.p,
-                        c->rho.p, listsOf(c), fresh, c->passIdx, int(c->passIdx > 1),
+                        c->rho.p, listsOf(c), fresh, c->passIdx, rebuild,
                         int(c->n > size_t(kNbrBias) + 1), (const uint2*)c->Q.p, c->QL.p,
                         c->LAM.p);
         });
