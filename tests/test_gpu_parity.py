@@ -301,6 +301,32 @@ def test_density_mc_bit_exact(ctx):
     _eq(tris["v"], otris["v"], "triangle vertices")
 
 
+@pytest.mark.parametrize("case", ["masses", "obstacles", "written"])
+def test_density_field_other_masses_bit_exact(ctx, case):
+    """The density field when the particles do not share one mass (its
+    16-byte staging records), when obstacles are present (their p* record
+    carries mass 0), and after positions were written with lf_write (which
+    may change masses, so the field no longer assumes one)."""
+    sc = scenes.dam_break_8k()
+    p = sc.params()
+    mass, typ, pos = sc.mass.copy(), sc.type.copy(), sc.position.copy()
+    if case == "masses":
+        mass = (1.0 + 0.25 * (np.arange(sc.n) % 4)).astype(np.float32)
+    elif case == "obstacles":
+        typ[::97] = lf.LF_OBSTACLE
+    ctx.upload(pos, sc.velocity, mass, typ, sc.colour)
+    o = oracle.Pbf(p)
+    o.load(pos, sc.velocity, mass, typ, sc.colour)
+    ctx.step(p, 1)
+    o.step(1)
+    if case == "written":
+        x = ctx.read(LF_ARR_POSITION)
+        ctx.write(LF_ARR_POSITION, x)  # the same bytes back
+    m = sc.mc()
+    ctx.mc_sample(p, m)
+    _eq(ctx.read(LF_ARR_FIELD).astype(np.float64), o.mc_density(m.samples), "density field")
+
+
 def test_reference_mc_bit_exact(ctx):
     sc = scenes.dam_break_8k()
     p = sc.params(flags=lf.LF_FLAG_DIFFUSE)
